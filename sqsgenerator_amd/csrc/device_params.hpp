@@ -1,0 +1,75 @@
+// Plain-old-data parameter blocks shared by the host driver and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace sqsg {
+
+constexpr int kMaxProblems = 4;   // split-mode sublattices handled in one candidate
+constexpr int kMaxBounds = 8;     // shuffle windows per problem (interact sublattices)
+constexpr int kMaxLevels = 16;    // distinct objective levels tracked on device (keep + 1)
+constexpr int kMaxLocalSpecies = 16;
+constexpr uint32_t kPairIndexBits = 12;  // packed pair: i | j << 12 | shell << 24
+constexpr uint32_t kMaxSites = 1u << kPairIndexBits;
+
+// One sublattice problem as the kernels see it (optimization_config_data, device side).
+struct DevProblem {
+  const uint32_t *pairs;      // P packed pairs, i < j in sorted-site order
+  const uint8_t *species0;    // N packed species (sorted configuration), padded to 4 bytes
+  const double *pref;         // [S][K][K]
+  const double *pw;           // [S][K][K] scaled by the shell weights
+  const double *tgt;          // [S][K][K]
+  const uint32_t *terms;      // T objective terms in reference order: s | xi << 8 | eta << 16
+  const uint32_t *nbr_start;  // N + 1 CSR offsets of the upper neighbour lists
+  const uint32_t *nbr;        // P entries j | shell << 16 (per site i, j > i)
+  const int8_t *adj;          // dense: [S][Npad][Npad] upper-triangular 0/1 (transposed: [s][j][i])
+  uint32_t N, K, S, P, T;
+  uint32_t Npad;              // N rounded up to the dense tile (32)
+  uint32_t nbounds;
+  uint32_t bounds[2 * kMaxBounds];
+  uint32_t draws;             // RNG draws per candidate: sum over windows of (w - 1)
+  uint32_t row_stride;        // bytes of one LDS candidate row (N rounded up, bank-padded)
+  uint64_t seed;
+  // systematic enumeration of window bounds[0..1]
+  uint32_t sys_nspecies;
+  uint32_t sys_hist[kMaxLocalSpecies];
+  uint8_t sys_rev[kMaxLocalSpecies];
+};
+
+struct RunParams {
+  DevProblem prob[kMaxProblems];
+  uint32_t nprob;
+  uint32_t systematic;
+  uint32_t chunk_quirk;        // interact MC: candidate t with t % chunk == 0 is unshuffled
+  uint32_t rounds;             // systematic: consecutive ranks walked per lane
+  uint64_t chunk_size;
+  uint64_t cand_begin, cand_end;
+  uint32_t L;                  // distinct objective levels kept (keep + 1)
+  uint32_t gen_group;          // candidates generated per wave and round (<= 64)
+  // LDS layout of one wave (bytes)
+  uint32_t wave_lds;
+  uint32_t off_rows[kMaxProblems];
+  uint32_t off_hist, off_terms, off_obj;
+  // tie-keeping threshold and record sink
+  unsigned long long *thr_key;
+  unsigned int *rec_count;
+  uint64_t *rec_cand;
+  double *rec_obj;
+  double *rec_sub;             // nprob objectives per record
+  uint8_t *rec_species;        // rec_stride bytes per record
+  uint32_t rec_cap;
+  uint32_t rec_stride;
+  unsigned long long *scored;
+};
+
+struct ScoreParams {
+  DevProblem prob;
+  const uint8_t *species;      // B x N
+  uint32_t B;
+  int32_t *bonds;              // B x S x K x K or null
+  double *objective;           // B
+  double *sro;                 // B x S x K x K or null
+  uint32_t wave_lds, off_row, off_hist, off_terms;
+};
+
+}  // namespace sqsg

@@ -1,0 +1,644 @@
+// Job construction (optimization_config<T,Mode>::from_config), the run driver (the
+// worker loop of optimizer::run, sqs.h:299-580, moved onto the device) and result
+// assembly (sqs_result_collection / postprocess_results / sqs_result_wrapper::rank).
+#include "job.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <random>
+#include <set>
+
+#include "kernels.hpp"
+#include "permutation.hpp"
+
+namespace sqsg {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::alloc(size_t bytes) {
+  reset();
+  if (bytes == 0) bytes = 16;
+  void *p = nullptr;
+  hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  p_ = p;
+  n_ = bytes;
+}
+
+void DevBuf::reset() {
+  if (p_) (void)hipFree(p_);
+  p_ = nullptr;
+  n_ = 0;
+}
+
+int device_cu_count() {
+  static int cus = [] {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    return prop.multiProcessorCount;
+  }();
+  return cus;
+}
+
+namespace {
+
+template <class T> void upload(DevBuf &b, const std::vector<T> &v, size_t pad_to = 1) {
+  const size_t bytes = v.size() * sizeof(T);
+  const size_t padded = ((bytes + pad_to - 1) / pad_to) * pad_to;
+  b.alloc(padded);
+  if (padded != bytes) SQSG_HIP(hipMemset(b.as<void>(), 0, padded));
+  if (bytes) SQSG_HIP(hipMemcpy(b.as<void>(), v.data(), bytes, hipMemcpyHostToDevice));
+}
+
+uint64_t make_random_seed() {  // shuffle.h:16-21
+  std::random_device rd;
+  std::mt19937_64 gen(rd());
+  std::uniform_int_distribution<uint64_t> dis;
+  return dis(gen);
+}
+
+struct ProblemInput {
+  std::vector<double> frac;          // n x 3
+  std::vector<uint8_t> z;            // n
+  std::vector<uint32_t> sort_order;  // sorted position -> input site
+  std::vector<uint32_t> bounds;
+  std::vector<uint32_t> sites;       // input site -> original structure site
+};
+
+void build_problem(Problem &P, const ProblemInput &in, const Lattice &lat, const sqsg_config &c,
+                   uint32_t p, bool systematic) {
+  const uint32_t n = static_cast<uint32_t>(in.z.size());
+  if (n == 0) throw std::invalid_argument("Cannot create a structure without atoms");
+  if (n > kMaxSites)
+    throw std::out_of_range("structures of more than 4096 sites are not supported on device");
+  P.N = n;
+  P.sort_order = in.sort_order;
+  P.bounds = in.bounds;
+  P.sites = in.sites;
+  // species map over the sorted structure (optimization_config.h:129-133)
+  std::set<uint8_t> zs(in.z.begin(), in.z.end());
+  P.zmap.assign(zs.begin(), zs.end());
+  P.K = static_cast<uint32_t>(P.zmap.size());
+  if (P.K != c.num_species[p])
+    throw std::invalid_argument("Incompatible configuration: species count does not match the "
+                                "extent of the parameter arrays");
+  uint8_t zidx[256];
+  for (uint32_t k = 0; k < P.K; ++k) zidx[P.zmap[k]] = static_cast<uint8_t>(k);
+  P.species_packed.resize(n);
+  for (uint32_t i = 0; i < n; ++i) P.species_packed[i] = zidx[in.z[i]];
+  // shells and weights (shell_weights_t is an ordered map)
+  const uint32_t s0 = c.shell_offsets[p], s1 = c.shell_offsets[p + 1];
+  if (s1 <= s0) throw std::out_of_range("no coordination shells selected");
+  P.S = s1 - s0;
+  if (P.S > 255) throw std::out_of_range("at most 255 coordination shells are supported");
+  P.shell_ids.assign(c.shells + s0, c.shells + s1);
+  for (uint32_t s = 1; s < P.S; ++s)
+    if (P.shell_ids[s] <= P.shell_ids[s - 1])
+      throw std::invalid_argument("shell indices must be strictly ascending");
+  std::vector<double> weights(c.shell_weights + s0, c.shell_weights + s1);
+  std::vector<double> radii(c.radii + c.radii_offsets[p], c.radii + c.radii_offsets[p + 1]);
+  // pair list of the sorted structure (structure.h:434-451; optimization_config.h:134)
+  const std::vector<double> dm = distance_matrix(lat, in.frac.data(), n);
+  const std::vector<uint32_t> sm = shell_matrix(dm, n, radii, kEps, 1.0e-9);
+  std::map<uint32_t, uint32_t> shell_index;
+  for (uint32_t s = 0; s < P.S; ++s) shell_index[P.shell_ids[s]] = s;
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = i + 1; j < n; ++j) {
+      auto it = shell_index.find(sm[size_t(i) * n + j]);
+      if (it == shell_index.end()) continue;
+      P.pi.push_back(i);
+      P.pj.push_back(j);
+      P.ps.push_back(it->second);
+    }
+  // parameter cubes [S][K][K]; pair weights scaled per shell (optimization.h:17-33)
+  size_t cube_off = 0;
+  for (uint32_t q = 0; q < p; ++q) {
+    const size_t Sq = c.shell_offsets[q + 1] - c.shell_offsets[q];
+    cube_off += Sq * c.num_species[q] * c.num_species[q];
+  }
+  const size_t cube = size_t(P.S) * P.K * P.K;
+  P.pref.assign(c.prefactors + cube_off, c.prefactors + cube_off + cube);
+  P.pw.assign(c.pair_weights + cube_off, c.pair_weights + cube_off + cube);
+  P.tgt.assign(c.target_objective + cube_off, c.target_objective + cube_off + cube);
+  const uint32_t K = P.K;
+  for (uint32_t s = 0; s < P.S; ++s)
+    for (uint32_t xi = 0; xi < K; ++xi)
+      for (uint32_t eta = xi; eta < K; ++eta) {
+        P.pw[(s * K + xi) * K + eta] *= weights[s];
+        if (xi == eta) continue;
+        P.pw[(s * K + eta) * K + xi] *= weights[s];
+      }
+  // objective terms in the order of optimization.h:85-100
+  for (uint32_t s = 0; s < P.S; ++s)
+    for (uint32_t xi = 0; xi < K; ++xi)
+      for (uint32_t eta = xi; eta < K; ++eta) P.terms.push_back(s | (xi << 8) | (eta << 16));
+  P.seed = c.seed_present && c.seed_present[p] ? c.seeds[p] : make_random_seed();
+  P.draws = 0;
+  for (size_t b = 0; b + 1 < P.bounds.size(); b += 2) {
+    if (P.bounds[b + 1] <= P.bounds[b]) throw std::invalid_argument("empty shuffle window");
+    P.draws += P.bounds[b + 1] - P.bounds[b] - 1;
+  }
+  if (P.bounds.size() / 2 > static_cast<size_t>(kMaxBounds))
+    throw std::out_of_range("at most 8 sublattices are supported in interact mode");
+  if (systematic) {
+    const uint32_t lo = P.bounds[0], hi = P.bounds[1];
+    std::set<uint8_t> local(P.species_packed.begin() + lo, P.species_packed.begin() + hi);
+    if (local.size() > static_cast<size_t>(kMaxLocalSpecies))
+      throw std::out_of_range("at most 16 species per systematic window");
+    P.sys_nspecies = 0;
+    for (uint8_t v : local) {
+      P.sys_rev[P.sys_nspecies] = v;
+      P.sys_hist[P.sys_nspecies] = static_cast<uint32_t>(
+          std::count(P.species_packed.begin() + lo, P.species_packed.begin() + hi, v));
+      ++P.sys_nspecies;
+    }
+  }
+}
+
+}  // namespace
+
+void Problem::upload_device() {
+  std::vector<uint32_t> packed(pi.size());
+  for (size_t q = 0; q < packed.size(); ++q)
+    packed[q] = pi[q] | (pj[q] << kPairIndexBits) | (ps[q] << 24);
+  upload(d_pairs, packed);
+  upload(d_species0, species_packed, 16);
+  upload(d_pref, pref);
+  upload(d_pw, pw);
+  upload(d_tgt, tgt);
+  upload(d_terms, terms);
+}
+
+void ensure_device(sqsg_job *job) {
+  if (job->on_device) return;
+  for (auto &P : job->problems) P.upload_device();
+  SQSG_HIP(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking));
+  SQSG_HIP(hipEventCreate(&job->ev0));
+  SQSG_HIP(hipEventCreate(&job->ev1));
+  job->on_device = true;
+}
+
+DevProblem Problem::dev() const {
+  DevProblem d{};
+  d.pairs = d_pairs.as<const uint32_t>();
+  d.species0 = d_species0.as<const uint8_t>();
+  d.pref = d_pref.as<const double>();
+  d.pw = d_pw.as<const double>();
+  d.tgt = d_tgt.as<const double>();
+  d.terms = d_terms.as<const uint32_t>();
+  d.nbr_start = d_nbr_start.as<const uint32_t>();
+  d.nbr = d_nbr.as<const uint32_t>();
+  d.adj = d_adj.as<const int8_t>();
+  d.N = N, d.K = K, d.S = S;
+  d.P = static_cast<uint32_t>(pi.size());
+  d.T = static_cast<uint32_t>(terms.size());
+  d.Npad = Npad;
+  d.nbounds = static_cast<uint32_t>(bounds.size() / 2);
+  for (size_t b = 0; b < bounds.size(); ++b) d.bounds[b] = bounds[b];
+  d.draws = draws;
+  d.row_stride = row_stride();
+  d.seed = seed;
+  d.sys_nspecies = sys_nspecies;
+  for (int j = 0; j < kMaxLocalSpecies; ++j) d.sys_hist[j] = sys_hist[j], d.sys_rev[j] = sys_rev[j];
+  return d;
+}
+
+sqsg_job *create_job(const sqsg_config &c) {
+  if (c.num_sites == 0 || !c.lattice || !c.frac_coords || !c.species)
+    throw std::invalid_argument("Cannot create a structure without atoms");
+  const bool systematic = c.iteration_mode == SQSG_ITERATION_SYSTEMATIC;
+  const bool split = c.sublattice_mode == SQSG_SUBLATTICE_SPLIT;
+  if ((c.iteration_mode != SQSG_ITERATION_RANDOM && !systematic) ||
+      (c.sublattice_mode != SQSG_SUBLATTICE_INTERACT && !split) || (systematic && split))
+    throw std::runtime_error("Invalid configuration of iteration and sublattice mode");
+  const uint32_t N = c.num_sites, nsub = c.num_sublattices;
+  if (nsub == 0) throw std::invalid_argument("You need to specify a composition");
+  if (systematic && nsub != 1)
+    throw std::invalid_argument("In \"systematic\" iteration mode only one sublattice is allowed");
+  if (c.keep == 0)
+    throw std::invalid_argument("The number of best kept structures must be greater than 0");
+  if (c.keep + 1 > static_cast<uint64_t>(kMaxLevels))
+    throw std::out_of_range("keep above 15 distinct objectives is not supported on device");
+  if (c.iterations == 0) throw std::invalid_argument("iterations must be positive");
+  if (c.chunk_size == 0 || c.chunk_size > c.iterations)
+    throw std::out_of_range("\"chunk_size\" must be in [1, iterations]");
+  auto job = std::make_unique<sqsg_job>();
+  job->iteration_mode = c.iteration_mode;
+  job->sublattice_mode = c.sublattice_mode;
+  job->num_sites = N;
+  job->iterations = c.iterations;
+  job->chunk_size = c.chunk_size;
+  job->keep = c.keep;
+  job->max_results = c.max_results_per_objective;
+  Lattice lat;
+  std::memcpy(lat.m, c.lattice, sizeof(lat.m));
+  // apply_composition (structure.h:390-399): sites in ascending order, species by Z
+  std::vector<std::vector<uint32_t>> sub_sites(nsub);
+  std::vector<uint32_t> owner(N, nsub);
+  job->structure_z.assign(c.species, c.species + N);
+  for (uint32_t q = 0; q < nsub; ++q) {
+    sub_sites[q].assign(c.sites + c.site_offsets[q], c.sites + c.site_offsets[q + 1]);
+    if (sub_sites[q].empty()) throw std::out_of_range("You cannot specify an empty slice");
+    for (size_t k = 0; k < sub_sites[q].size(); ++k) {
+      const uint32_t s = sub_sites[q][k];
+      if (s >= N) throw std::out_of_range("site index out of range");
+      if (k && s <= sub_sites[q][k - 1]) throw std::invalid_argument("sites must be ascending");
+      if (owner[s] != nsub)
+        throw std::invalid_argument("The site is contained in more than one sublattice");
+      owner[s] = q;
+    }
+    size_t idx = 0, total = 0;
+    for (uint32_t e = c.comp_offsets[q]; e < c.comp_offsets[q + 1]; ++e) total += c.comp_amounts[e];
+    if (total != sub_sites[q].size())
+      throw std::out_of_range("The total number of distributed atoms does not match the sites");
+    for (uint32_t e = c.comp_offsets[q]; e < c.comp_offsets[q + 1]; ++e)
+      for (uint32_t a = 0; a < c.comp_amounts[e]; ++a)
+        job->structure_z[sub_sites[q][idx++]] = c.comp_species[e];
+  }
+  const uint32_t nprob = split ? nsub : 1;
+  if (nprob > static_cast<uint32_t>(kMaxProblems))
+    throw std::out_of_range("split mode supports at most 4 sublattices");
+  job->problems.resize(nprob);
+  if (!split) {
+    // compute_shuffling_bounds (optimization.h:35-72): std::sort by sublattice index
+    struct Site {
+      uint32_t index, sl;
+    };
+    std::vector<Site> sites(N);
+    for (uint32_t i = 0; i < N; ++i) sites[i] = {i, owner[i]};
+    std::sort(sites.begin(), sites.end(),
+              [](const Site &a, const Site &b) { return a.sl < b.sl; });
+    ProblemInput in;
+    in.frac.resize(3 * size_t(N));
+    in.z.resize(N);
+    in.sort_order.resize(N);
+    in.sites.resize(N);
+    for (uint32_t k = 0; k < N; ++k) {
+      const uint32_t o = sites[k].index;
+      in.sort_order[k] = o;
+      in.sites[k] = k;
+      in.z[k] = job->structure_z[o];
+      for (int d = 0; d < 3; ++d) in.frac[3 * k + d] = c.frac_coords[3 * size_t(o) + d];
+    }
+    uint32_t lower = 0;
+    for (uint32_t i = 0; i < N; ++i) {
+      if (sites[i].sl != sites[lower].sl) {
+        in.bounds.push_back(lower);
+        in.bounds.push_back(i);
+        lower = i;
+        if (sites[i].sl == nsub) break;
+      } else if (i == N - 1) {
+        in.bounds.push_back(lower);
+        in.bounds.push_back(i + 1);
+      }
+    }
+    if (in.bounds.size() / 2 != nsub) throw std::invalid_argument("Incompatible configuration");
+    build_problem(job->problems[0], in, lat, c, 0, systematic);
+  } else {
+    for (uint32_t q = 0; q < nsub; ++q) {
+      ProblemInput in;
+      const uint32_t n = static_cast<uint32_t>(sub_sites[q].size());
+      in.frac.resize(3 * size_t(n));
+      in.z.resize(n);
+      in.sort_order.resize(n);
+      for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t o = sub_sites[q][k];
+        in.sort_order[k] = k;
+        in.z[k] = job->structure_z[o];
+        for (int d = 0; d < 3; ++d) in.frac[3 * k + d] = c.frac_coords[3 * size_t(o) + d];
+      }
+      in.sites = sub_sites[q];
+      in.bounds = {0, n};
+      build_problem(job->problems[q], in, lat, c, q, false);
+    }
+  }
+  if (systematic) {
+    const Problem &P = job->problems[0];
+    const BigUint total =
+        num_permutations(P.species_packed.data() + P.bounds[0], P.bounds[1] - P.bounds[0]);
+    if (!total.fits_u64())
+      throw std::out_of_range("The number of permutations to test is " + total.str() +
+                              ", more than 2^64 - 1");
+    if (static_cast<u128>(c.iterations) > total.to_u128())
+      throw std::out_of_range("iterations exceed the number of permutations");
+  }
+  return job.release();
+}
+
+RunParams make_run_params(sqsg_job *job, uint32_t W, uint32_t *blocks) {
+  RunParams p{};
+  p.nprob = static_cast<uint32_t>(job->problems.size());
+  uint32_t stride_sum = 0, max_bins = 0, max_terms = 0;
+  for (uint32_t q = 0; q < p.nprob; ++q) {
+    const Problem &P = job->problems[q];
+    p.prob[q] = P.dev();
+    stride_sum += P.row_stride();
+    max_bins = std::max(max_bins, P.S * P.K * P.K);
+    max_terms = std::max(max_terms, static_cast<uint32_t>(P.terms.size()));
+  }
+  p.systematic = job->iteration_mode == SQSG_ITERATION_SYSTEMATIC;
+  p.chunk_quirk = (!p.systematic && job->sublattice_mode == SQSG_SUBLATTICE_INTERACT) ? 1u : 0u;
+  p.chunk_size = job->chunk_size;
+  p.L = static_cast<uint32_t>(job->keep + 1);
+  uint32_t G = 64;
+  while (G > 8 && static_cast<size_t>(G) * stride_sum > 24576) G >>= 1;
+  p.gen_group = G;
+  auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
+  uint32_t off = 0;
+  for (uint32_t q = 0; q < p.nprob; ++q) {
+    p.off_rows[q] = off;
+    off = align(off + G * job->problems[q].row_stride());
+  }
+  p.off_hist = off;
+  off = align(off + 4 * max_bins);
+  p.off_terms = off;
+  off = align(off + 8 * max_terms);
+  p.off_obj = off;
+  off = align(off + 8 * 64);
+  p.wave_lds = off;
+  const size_t block_lds = static_cast<size_t>(W) * p.wave_lds;
+  if (block_lds > 160 * 1024) throw std::out_of_range("problem does not fit the LDS budget");
+  uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / block_lds);
+  per_cu = std::max(1u, std::min(per_cu, 8u / std::max(1u, W / 2)));
+  *blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
+  return p;
+}
+
+void merge_records(sqsg_job *job, std::vector<Record> &&fresh) {
+  auto &rec = job->records;
+  for (auto &r : fresh) rec.push_back(std::move(r));
+  std::vector<double> objs;
+  objs.reserve(rec.size());
+  for (const auto &r : rec) objs.push_back(r.obj);
+  std::sort(objs.begin(), objs.end());
+  objs.erase(std::unique(objs.begin(), objs.end()), objs.end());
+  const size_t L = job->keep + 1;
+  const double v = objs.size() >= L ? objs[L - 1] : std::numeric_limits<double>::infinity();
+  job->vkeep = v;
+  rec.erase(std::remove_if(rec.begin(), rec.end(), [v](const Record &r) { return r.obj > v; }),
+            rec.end());
+}
+
+void finalize_results(sqsg_job *job) {
+  auto &rec = job->records;
+  std::sort(rec.begin(), rec.end(), [](const Record &a, const Record &b) {
+    return a.obj < b.obj || (a.obj == b.obj && a.cand < b.cand);
+  });
+  job->results.clear();
+  job->num_levels = 0;
+  const uint32_t nprob = static_cast<uint32_t>(job->problems.size());
+  for (size_t a = 0; a < rec.size();) {
+    size_t b = a;
+    while (b < rec.size() && rec[b].obj == rec[a].obj) ++b;
+    // one set per objective keyed on the configuration (results.h:46-53); with
+    // max_results_per_objective the reference inserts while count <= max (sqs.h:423-426)
+    std::set<std::vector<uint8_t>> seen;
+    for (size_t k = a; k < b; ++k) {
+      if (job->max_results >= 0 && seen.size() > static_cast<size_t>(job->max_results)) break;
+      if (!seen.insert(rec[k].species).second) continue;
+      Result r;
+      r.objective = rec[k].obj;
+      r.sub.assign(rec[k].sub, rec[k].sub + nprob);
+      r.packed = rec[k].species;
+      r.cand = rec[k].cand;
+      job->results.push_back(std::move(r));
+    }
+    ++job->num_levels;
+    a = b;
+  }
+  if (job->results.empty()) return;
+  // SRO cubes of the reported configurations, scored on the device
+  const size_t R = job->results.size();
+  size_t off = 0;
+  for (uint32_t q = 0; q < nprob; ++q) {
+    const Problem &P = job->problems[q];
+    const size_t nb = size_t(P.S) * P.K * P.K;
+    std::vector<uint8_t> sp(R * P.N);
+    for (size_t r = 0; r < R; ++r)
+      std::memcpy(&sp[r * P.N], job->results[r].packed.data() + off, P.N);
+    DevBuf d_sp(sp.size()), d_obj(R * sizeof(double)), d_sro(R * nb * sizeof(double));
+    SQSG_HIP(hipMemcpy(d_sp.as<void>(), sp.data(), sp.size(), hipMemcpyHostToDevice));
+    score_batch(job, q, d_sp.as<uint8_t>(), static_cast<uint32_t>(R), nullptr,
+                d_obj.as<double>(), d_sro.as<double>(), SQSG_KERNEL_SPARSE_WAVE, job->stream);
+    std::vector<double> obj(R), sro(R * nb);
+    SQSG_HIP(hipMemcpyAsync(obj.data(), d_obj.as<void>(), R * sizeof(double),
+                            hipMemcpyDeviceToHost, job->stream));
+    SQSG_HIP(hipMemcpyAsync(sro.data(), d_sro.as<void>(), sro.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, job->stream));
+    SQSG_HIP(hipStreamSynchronize(job->stream));
+    for (size_t r = 0; r < R; ++r) {
+      if (obj[r] != job->results[r].sub[q])
+        throw DeviceError("re-scored objective differs from the recorded one");
+      job->results[r].sro.insert(job->results[r].sro.end(), sro.begin() + r * nb,
+                                 sro.begin() + (r + 1) * nb);
+    }
+    off += P.N;
+  }
+  // original site order and atomic numbers (sqs.h:94-122), reported ranks (results.h:163-169)
+  for (auto &r : job->results) {
+    r.species_z = job->structure_z;
+    if (job->sublattice_mode == SQSG_SUBLATTICE_INTERACT) {
+      const Problem &P = job->problems[0];
+      for (uint32_t k = 0; k < P.N; ++k) r.species_z[P.sort_order[k]] = P.zmap[r.packed[k]];
+      uint8_t zidx[256] = {0};
+      for (uint32_t k = 0; k < P.K; ++k) zidx[P.zmap[k]] = static_cast<uint8_t>(k);
+      std::vector<uint8_t> orig(P.N);
+      for (uint32_t i = 0; i < P.N; ++i) orig[i] = zidx[r.species_z[i]];
+      r.rank = rank_permutation(orig.data(), orig.size()).str();
+    } else {
+      size_t o = 0;
+      std::string ranks;
+      for (const auto &P : job->problems) {
+        for (uint32_t k = 0; k < P.N; ++k) r.species_z[P.sites[k]] = P.zmap[r.packed[o + k]];
+        if (!ranks.empty()) ranks += ",";
+        ranks += rank_permutation(r.packed.data() + o, P.N).str();
+        o += P.N;
+      }
+      r.rank = ranks;
+    }
+  }
+}
+
+namespace {
+
+struct RecordSink {
+  DevBuf cand, obj, sub, species;
+  uint32_t cap = 0;
+  void ensure(uint32_t n, uint32_t stride, uint32_t nprob) {
+    if (n <= cap) return;
+    cap = n;
+    cand.alloc(size_t(n) * sizeof(uint64_t));
+    obj.alloc(size_t(n) * sizeof(double));
+    sub.alloc(size_t(n) * nprob * sizeof(double));
+    species.alloc(size_t(n) * stride);
+  }
+};
+
+uint64_t default_launch_size(const sqsg_job *job) {
+  uint64_t work = 0;
+  for (const auto &P : job->problems) work += P.pi.size() + 64;
+  uint64_t launch = (uint64_t(1) << 34) / std::max<uint64_t>(work, 1);
+  return std::clamp<uint64_t>(launch, uint64_t(1) << 16, uint64_t(1) << 24);
+}
+
+}  // namespace
+
+void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
+  using clock = std::chrono::steady_clock;
+  const auto t_start = clock::now();
+  ensure_device(job);
+  job->cancel.store(false);
+  job->records.clear();
+  job->results.clear();
+  job->vkeep = std::numeric_limits<double>::infinity();
+  job->stats = sqsg_statistics{};
+  job->stats.best_objective = std::numeric_limits<double>::infinity();
+  job->kernel_ms = 0;
+  job->launches = 0;
+  job->scored = 0;
+  const uint32_t W = 4;
+  uint32_t blocks = 0;
+  RunParams p = make_run_params(job, W, &blocks);
+  const uint64_t launch = job->launch_size ? job->launch_size : default_launch_size(job);
+  if (p.systematic) {
+    const uint64_t lanes = uint64_t(blocks) * W * p.gen_group;
+    p.rounds = static_cast<uint32_t>(std::clamp<uint64_t>(launch / (lanes * 2), 1, 1024));
+  } else {
+    p.rounds = 1;
+  }
+  uint32_t stride = 0;
+  for (const auto &P : job->problems) stride += P.N;
+  p.rec_stride = stride;
+  RecordSink sink;
+  sink.ensure(static_cast<uint32_t>(std::min<uint64_t>(launch, 1u << 16)), stride, p.nprob);
+  DevBuf d_thr(sizeof(unsigned long long)), d_count(sizeof(unsigned int)),
+      d_scored(sizeof(unsigned long long));
+  unsigned long long thr = objective_key(std::numeric_limits<double>::infinity());
+  SQSG_HIP(hipMemcpy(d_thr.as<void>(), &thr, sizeof(thr), hipMemcpyHostToDevice));
+  p.thr_key = d_thr.as<unsigned long long>();
+  p.rec_count = d_count.as<unsigned int>();
+  p.scored = d_scored.as<unsigned long long>();
+  const auto t_loop = clock::now();
+  job->stats.timing_chunk_setup_ns =
+      std::chrono::duration_cast<std::chrono::nanoseconds>(t_loop - t_start).count();
+  uint64_t begin = 0;
+  while (begin < job->iterations) {
+    if (job->cancel.load()) break;
+    const uint64_t end = std::min(job->iterations, begin + launch);
+    p.cand_begin = begin;
+    p.cand_end = end;
+    p.rec_cap = sink.cap;
+    p.rec_cand = sink.cand.as<uint64_t>();
+    p.rec_obj = sink.obj.as<double>();
+    p.rec_sub = sink.sub.as<double>();
+    p.rec_species = sink.species.as<uint8_t>();
+    SQSG_HIP(hipMemsetAsync(d_count.as<void>(), 0, sizeof(unsigned int), job->stream));
+    SQSG_HIP(hipMemsetAsync(d_scored.as<void>(), 0, sizeof(unsigned long long), job->stream));
+    SQSG_HIP(hipEventRecord(job->ev0, job->stream));
+    SQSG_HIP(launch_run_wave(p, blocks, W, job->stream));
+    SQSG_HIP(hipEventRecord(job->ev1, job->stream));
+    unsigned int count = 0;
+    unsigned long long scored = 0;
+    SQSG_HIP(hipMemcpyAsync(&count, d_count.as<void>(), sizeof(count), hipMemcpyDeviceToHost,
+                            job->stream));
+    SQSG_HIP(hipMemcpyAsync(&scored, d_scored.as<void>(), sizeof(scored), hipMemcpyDeviceToHost,
+                            job->stream));
+    SQSG_HIP(hipStreamSynchronize(job->stream));
+    float ms = 0;
+    SQSG_HIP(hipEventElapsedTime(&ms, job->ev0, job->ev1));
+    job->kernel_ms += ms;
+    job->launches++;
+    if (count > sink.cap) {  // sink overflow: widen and run the same range again
+      sink.ensure(count + count / 4, stride, p.nprob);
+      continue;
+    }
+    if (scored != end - begin) throw DeviceError("device scored an unexpected candidate count");
+    job->scored += scored;
+    std::vector<Record> fresh(count);
+    if (count) {
+      std::vector<uint64_t> cand(count);
+      std::vector<double> obj(count), sub(size_t(count) * p.nprob);
+      std::vector<uint8_t> sp(size_t(count) * stride);
+      SQSG_HIP(hipMemcpy(cand.data(), sink.cand.as<void>(), count * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost));
+      SQSG_HIP(hipMemcpy(obj.data(), sink.obj.as<void>(), count * sizeof(double),
+                         hipMemcpyDeviceToHost));
+      SQSG_HIP(hipMemcpy(sub.data(), sink.sub.as<void>(), sub.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+      SQSG_HIP(hipMemcpy(sp.data(), sink.species.as<void>(), sp.size(), hipMemcpyDeviceToHost));
+      for (uint32_t k = 0; k < count; ++k) {
+        fresh[k].cand = cand[k];
+        fresh[k].obj = obj[k];
+        for (uint32_t q = 0; q < p.nprob; ++q) fresh[k].sub[q] = sub[size_t(k) * p.nprob + q];
+        fresh[k].species.assign(sp.begin() + size_t(k) * stride,
+                                sp.begin() + size_t(k + 1) * stride);
+      }
+    }
+    merge_records(job, std::move(fresh));
+    // keep the device threshold at the tighter of its own bound and the exact host one
+    const unsigned long long hk = objective_key(job->vkeep);
+    unsigned long long dk = 0;
+    SQSG_HIP(hipMemcpy(&dk, d_thr.as<void>(), sizeof(dk), hipMemcpyDeviceToHost));
+    if (hk < dk) SQSG_HIP(hipMemcpy(d_thr.as<void>(), &hk, sizeof(hk), hipMemcpyHostToDevice));
+    job->stats.finished = job->scored;
+    for (const auto &r : job->records)
+      if (r.obj < job->stats.best_objective ||
+          (r.obj == job->stats.best_objective && r.cand < job->stats.best_rank)) {
+        job->stats.best_objective = r.obj;
+        job->stats.best_rank = r.cand;
+      }
+    begin = end;
+    if (cb) {
+      const auto now = clock::now();
+      job->stats.timing_loop_ns =
+          std::chrono::duration_cast<std::chrono::nanoseconds>(now - t_loop).count();
+      if (cb(&job->stats, user)) job->cancel.store(true);
+    }
+  }
+  finalize_results(job);
+  const auto t_end = clock::now();
+  job->stats.timing_loop_ns =
+      std::chrono::duration_cast<std::chrono::nanoseconds>(t_end - t_loop).count();
+  job->stats.timing_total_ns =
+      std::chrono::duration_cast<std::chrono::nanoseconds>(t_end - t_start).count();
+}
+
+void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint32_t B,
+                 int32_t *d_bonds, double *d_obj, double *d_sro, int kernel, hipStream_t s) {
+  if (problem >= job->problems.size()) throw std::out_of_range("problem index out of range");
+  ensure_device(job);
+  if (kernel != SQSG_KERNEL_AUTO && kernel != SQSG_KERNEL_SPARSE_WAVE)
+    throw std::invalid_argument("kernel variant not available for explicit scoring");
+  const Problem &P = job->problems[problem];
+  ScoreParams sp{};
+  sp.prob = P.dev();
+  sp.species = d_species;
+  sp.B = B;
+  sp.bonds = d_bonds;
+  sp.objective = d_obj;
+  sp.sro = d_sro;
+  auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
+  sp.off_row = 0;
+  sp.off_hist = align(P.row_stride());
+  sp.off_terms = align(sp.off_hist + 4 * P.S * P.K * P.K);
+  sp.wave_lds = align(sp.off_terms + 8 * static_cast<uint32_t>(P.terms.size()));
+  if (size_t(4) * sp.wave_lds > 160 * 1024) throw std::out_of_range("problem exceeds LDS");
+  SQSG_HIP(launch_score_explicit(sp, s ? s : job->stream));
+}
+
+}  // namespace sqsg
+
+sqsg_job::~sqsg_job() {
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}

@@ -1,0 +1,138 @@
+// Host-side job: optimization_config (include/sqsgen/core/optimization_config.h) plus
+// the device-resident copies of its read-only inputs, the run state and the results.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sqsgen_amd.h"
+#include "device_params.hpp"
+#include "geometry.hpp"
+
+namespace sqsg {
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct CancelledError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what);
+#define SQSG_HIP(call) ::sqsg::hip_check((call), #call)
+
+// Owning device allocation.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { alloc(bytes); }
+  ~DevBuf() { reset(); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr, o.n_ = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      reset();
+      p_ = o.p_, n_ = o.n_;
+      o.p_ = nullptr, o.n_ = 0;
+    }
+    return *this;
+  }
+  void alloc(size_t bytes);
+  void reset();
+  template <class T> T *as() const { return static_cast<T *>(p_); }
+  size_t size() const { return n_; }
+
+ private:
+  void *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// One sublattice problem (interact: the whole structure; split: one sublattice).
+struct Problem {
+  uint32_t N = 0, K = 0, S = 0;
+  std::vector<uint32_t> pi, pj, ps;     // pairs in the reference's build order (i, then j)
+  std::vector<uint8_t> species_packed;  // sorted-site order, packed indices
+  std::vector<uint32_t> sort_order;     // sorted position -> site of the problem's structure
+  std::vector<uint32_t> bounds;         // (lo, hi) shuffle windows
+  std::vector<uint8_t> zmap;            // packed index -> Z
+  std::vector<uint32_t> shell_ids;      // packed shell -> shell id
+  std::vector<double> pref, pw, tgt;    // [S][K][K]; pw scaled by the shell weights
+  std::vector<uint32_t> terms;          // objective terms in reference order
+  std::vector<uint32_t> sites;          // problem site -> original structure site
+  uint64_t seed = 0;
+  uint32_t draws = 0;
+  // systematic window
+  uint32_t sys_nspecies = 0;
+  uint32_t sys_hist[kMaxLocalSpecies] = {};
+  uint8_t sys_rev[kMaxLocalSpecies] = {};
+  // device copies
+  DevBuf d_pairs, d_species0, d_pref, d_pw, d_tgt, d_terms, d_nbr_start, d_nbr, d_adj;
+  uint32_t Npad = 0;
+  DevProblem dev() const;
+  void upload_device();
+  uint32_t row_stride() const { return (((N + 3) / 4) | 1u) * 4u; }
+};
+
+struct Record {
+  uint64_t cand;
+  double obj;
+  double sub[kMaxProblems];
+  std::vector<uint8_t> species;  // concatenated per problem, sorted-site packed
+};
+
+struct Result {
+  double objective;
+  std::vector<double> sub;
+  std::vector<uint8_t> packed;     // concatenated per problem
+  std::vector<uint8_t> species_z;  // whole structure, original order
+  std::vector<double> sro;         // concatenated [S][K][K] per problem
+  std::string rank;
+  uint64_t cand;
+};
+
+}  // namespace sqsg
+
+struct sqsg_job {
+  int iteration_mode = 0, sublattice_mode = 0;
+  std::vector<sqsg::Problem> problems;
+  uint32_t num_sites = 0;
+  std::vector<uint8_t> structure_z;  // applied composition, original order
+  uint64_t iterations = 0, chunk_size = 0, keep = 1;
+  int64_t max_results = -1;
+  int kernel = SQSG_KERNEL_AUTO;
+  uint64_t launch_size = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool on_device = false;
+  std::atomic<bool> cancel{false};
+  // run state
+  std::vector<sqsg::Record> records;
+  double vkeep = 0;
+  sqsg_statistics stats{};
+  double kernel_ms = 0;
+  uint64_t launches = 0, scored = 0;
+  std::vector<sqsg::Result> results;
+  uint64_t num_levels = 0;
+  ~sqsg_job();
+};
+
+namespace sqsg {
+
+sqsg_job *create_job(const sqsg_config &cfg);
+void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user);
+void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint32_t B,
+                 int32_t *d_bonds, double *d_obj, double *d_sro, int kernel, hipStream_t s);
+// helpers shared with the batched and serving drivers
+RunParams make_run_params(sqsg_job *job, uint32_t waves_per_block, uint32_t *blocks);
+void merge_records(sqsg_job *job, std::vector<Record> &&fresh);
+void finalize_results(sqsg_job *job);
+int device_cu_count();
+void ensure_device(sqsg_job *job);
+
+}  // namespace sqsg

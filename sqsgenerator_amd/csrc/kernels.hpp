@@ -1,0 +1,31 @@
+// Host-callable launchers of the HIP kernels (kernels.hip, dense.hip, batch.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "device_params.hpp"
+
+namespace sqsg {
+
+// explicit candidates -> ordered int32 bonds, FP64 objective, optional SRO cube
+hipError_t launch_score_explicit(const ScoreParams &p, hipStream_t stream);
+
+// generation (MC shuffle / systematic unrank+step) + scoring + tie-keeping threshold,
+// one wavefront per candidate (the plain single-job kernel)
+hipError_t launch_run_wave(const RunParams &p, uint32_t blocks, uint32_t waves_per_block,
+                           hipStream_t stream);
+
+// order-preserving key of a double (unsigned order == numeric order)
+inline unsigned long long objective_key(double d) {
+  unsigned long long b;
+  __builtin_memcpy(&b, &d, 8);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+inline double key_objective(unsigned long long k) {
+  unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  double d;
+  __builtin_memcpy(&d, &b, 8);
+  return d;
+}
+
+}  // namespace sqsg
