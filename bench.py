@@ -64,7 +64,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--candidates-per-step", type=int, default=1 << 20)
-    ap.add_argument("--kernel", default="auto", choices=["auto", "wave"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "wave", "lane"])
+    ap.add_argument("--plain-steps", type=int, default=3,
+                    help="launches of the plain wave-per-candidate kernel timed for comparison")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -81,14 +83,23 @@ def main() -> int:
     from sqsgenerator_amd.core import Job, parse_config
 
     per_step = args.candidates_per_step
-    kernel = {"auto": L.KERNEL_AUTO, "wave": L.KERNEL_SPARSE_WAVE}[args.kernel]
+    kernel = {"auto": L.KERNEL_AUTO, "wave": L.KERNEL_SPARSE_WAVE,
+              "lane": L.KERNEL_SPARSE_LANE}[args.kernel]
 
-    def make(iterations):
+    def make(iterations, k=kernel):
         cfg = parse_config(W.quinary_256(iterations=iterations, seed=7))
         job = Job(cfg)
-        job.set_kernel(kernel)
+        job.set_kernel(k)
         job.set_launch_size(per_step)
         return cfg, job
+
+    plain = None
+    if args.plain_steps:
+        _, pjob = make(args.plain_steps * per_step, L.KERNEL_SPARSE_WAVE)
+        pjob.run()
+        pms, _, pn = pjob.timing()
+        plain = pn / (pms / 1e3)
+        pjob.close()
 
     # warmup: W launches of the same job
     if args.warmup:
@@ -106,6 +117,7 @@ def main() -> int:
     kernel_ms, launches, scored = job.timing()
     assert scored == args.steps * per_step, (scored, args.steps * per_step)
     value = scored / wall
+    kernel_name = {L.KERNEL_SPARSE_WAVE: "run_wave_kernel"}.get(kernel, "run_lane_kernel<5>")
     avg_launch_s = kernel_ms / 1e3 / launches
     P, N = info["num_pairs"], info["num_sites"]
     bytes_per_cand = 6 * P + N + 8
@@ -118,10 +130,12 @@ def main() -> int:
         "config": {"workload": "fcc4x4x4-256site-quinary-7shell-mc", "sites": N,
                    "species": info["num_species"], "shells": info["num_shells"], "pairs": P,
                    "candidates_per_step": per_step, "kernel": args.kernel,
-                   "kernel_candidates_per_s": scored / (kernel_ms / 1e3)},
+                   "kernel_candidates_per_s": scored / (kernel_ms / 1e3),
+                   "plain_kernel_candidates_per_s": plain,
+                   "speedup_vs_plain": (scored / (kernel_ms / 1e3)) / plain if plain else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "run_wave_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+                     "kernel": kernel_name, "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_candidate": bytes_per_cand},
     }
     job.close()

@@ -194,6 +194,15 @@ SQSG_API int sqsg_result_get(sqsg_job *job, uint64_t r, double *objective, uint8
                              uint64_t *candidate);
 /* internal (sorted-site, packed) species of result r, concatenated per problem */
 SQSG_API int sqsg_result_species_packed(sqsg_job *job, uint64_t r, uint8_t *species);
+/* objective of each level (ascending) and the levels + 1 offsets of their results */
+SQSG_API int sqsg_result_levels(sqsg_job *job, double *objectives, uint64_t *level_start);
+/* bulk form of sqsg_result_get for packs with millions of ties: objectives, candidates,
+ * original-order species (count x num_sites) and ranks as uint64 (UINT64_MAX where the
+ * rank needs more than 64 bits or the job is in split mode); any pointer may be NULL */
+SQSG_API int sqsg_result_bulk(sqsg_job *job, uint64_t first, uint64_t count, double *objective,
+                              uint64_t *candidate, uint8_t *species_z, uint64_t *rank);
+/* SRO cubes of results [first, first + count), concatenated per problem */
+SQSG_API int sqsg_result_sro_bulk(sqsg_job *job, uint64_t first, uint64_t count, double *sro);
 
 /* ---- batched persistent form (no reference counterpart: it batches many
  *      run_optimization calls into one device launch) ------------------------------------ */

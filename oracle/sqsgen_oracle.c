@@ -448,10 +448,21 @@ static struct {
   or_entry *entries;
   uint64_t num_entries, cap_entries;
   double *levels;        /* sorted distinct objectives (results.h:97) */
+  uint64_t *level_count; /* set size per level (results.h:63-69) */
   uint64_t num_levels, cap_levels;
+  uint64_t *table;       /* open-addressing set of entry index + 1, keyed on (objective, species) */
+  uint64_t table_cap;
   uint32_t blob;         /* species bytes per entry */
   uint32_t nsub;
-} g_coll = {PTHREAD_MUTEX_INITIALIZER, 0, 0, 0, 0, 0, 0, 0, 0};
+} g_coll = {PTHREAD_MUTEX_INITIALIZER, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+static uint64_t entry_hash(double obj, const uint8_t *species, uint32_t blob) {
+  uint64_t h = 0xcbf29ce484222325ull, b;
+  memcpy(&b, &obj, 8);
+  h = (h ^ b) * 0x100000001b3ull;
+  for (uint32_t i = 0; i < blob; ++i) h = (h ^ species[i]) * 0x100000001b3ull;
+  return h ^ (h >> 29);
+}
 
 static void coll_reset(uint32_t blob, uint32_t nsub) {
   for (uint64_t e = 0; e < g_coll.num_entries; ++e) {
@@ -460,48 +471,85 @@ static void coll_reset(uint32_t blob, uint32_t nsub) {
   }
   free(g_coll.entries);
   free(g_coll.levels);
-  g_coll.entries = 0; g_coll.levels = 0;
+  free(g_coll.level_count);
+  free(g_coll.table);
+  g_coll.entries = 0; g_coll.levels = 0; g_coll.level_count = 0;
   g_coll.num_entries = g_coll.cap_entries = g_coll.num_levels = g_coll.cap_levels = 0;
+  g_coll.table_cap = 1024;
+  g_coll.table = (uint64_t *)calloc(g_coll.table_cap, sizeof(uint64_t));
   g_coll.blob = blob;
   g_coll.nsub = nsub;
 }
 
+static uint64_t level_pos(double obj) { /* first level >= obj */
+  uint64_t lo = 0, hi = g_coll.num_levels;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) / 2;
+    if (g_coll.levels[mid] < obj) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
 static uint64_t coll_count_for(double obj) { /* results.h:63-69 */
-  uint64_t c = 0;
-  for (uint64_t e = 0; e < g_coll.num_entries; ++e) c += (g_coll.entries[e].objective == obj);
-  return c;
+  uint64_t pos = level_pos(obj);
+  return (pos < g_coll.num_levels && g_coll.levels[pos] == obj) ? g_coll.level_count[pos] : 0;
 }
 
 static double coll_nth_best(uint64_t n) { /* results.h:78-82 */
   return n >= g_coll.num_levels ? INFINITY : g_coll.levels[n];
 }
 
+static void table_put(uint64_t idx) {
+  const or_entry *en = &g_coll.entries[idx];
+  uint64_t mask = g_coll.table_cap - 1;
+  uint64_t h = entry_hash(en->objective, en->species, g_coll.blob) & mask;
+  while (g_coll.table[h]) h = (h + 1) & mask;
+  g_coll.table[h] = idx + 1;
+}
+
 static void coll_insert(double obj, const uint8_t *species, const double *subobj) {
   /* results.h:46-53: a set per objective keyed on the species */
-  for (uint64_t e = 0; e < g_coll.num_entries; ++e)
-    if (g_coll.entries[e].objective == obj
-        && memcmp(g_coll.entries[e].species, species, g_coll.blob) == 0)
-      return;
+  uint64_t mask = g_coll.table_cap - 1;
+  uint64_t h = entry_hash(obj, species, g_coll.blob) & mask;
+  while (g_coll.table[h]) {
+    const or_entry *en = &g_coll.entries[g_coll.table[h] - 1];
+    if (en->objective == obj && memcmp(en->species, species, g_coll.blob) == 0) return;
+    h = (h + 1) & mask;
+  }
   if (g_coll.num_entries == g_coll.cap_entries) {
     g_coll.cap_entries = g_coll.cap_entries ? 2 * g_coll.cap_entries : 64;
     g_coll.entries = (or_entry *)realloc(g_coll.entries, sizeof(or_entry) * g_coll.cap_entries);
   }
-  or_entry *en = &g_coll.entries[g_coll.num_entries++];
+  uint64_t idx = g_coll.num_entries++;
+  or_entry *en = &g_coll.entries[idx];
   en->objective = obj;
   en->species = (uint8_t *)malloc(g_coll.blob);
   memcpy(en->species, species, g_coll.blob);
   en->sub_objectives = (double *)malloc(sizeof(double) * g_coll.nsub);
   memcpy(en->sub_objectives, subobj, sizeof(double) * g_coll.nsub);
-  uint64_t pos = 0;
-  while (pos < g_coll.num_levels && g_coll.levels[pos] < obj) pos++;
-  if (pos < g_coll.num_levels && g_coll.levels[pos] == obj) return;
+  g_coll.table[h] = idx + 1;
+  if (2 * g_coll.num_entries > g_coll.table_cap) {
+    free(g_coll.table);
+    g_coll.table_cap *= 2;
+    g_coll.table = (uint64_t *)calloc(g_coll.table_cap, sizeof(uint64_t));
+    for (uint64_t e = 0; e < g_coll.num_entries; ++e) table_put(e);
+  }
+  uint64_t pos = level_pos(obj);
+  if (pos < g_coll.num_levels && g_coll.levels[pos] == obj) {
+    g_coll.level_count[pos]++;
+    return;
+  }
   if (g_coll.num_levels == g_coll.cap_levels) {
     g_coll.cap_levels = g_coll.cap_levels ? 2 * g_coll.cap_levels : 64;
     g_coll.levels = (double *)realloc(g_coll.levels, sizeof(double) * g_coll.cap_levels);
+    g_coll.level_count = (uint64_t *)realloc(g_coll.level_count, sizeof(uint64_t) * g_coll.cap_levels);
   }
   memmove(g_coll.levels + pos + 1, g_coll.levels + pos,
           sizeof(double) * (g_coll.num_levels - pos));
+  memmove(g_coll.level_count + pos + 1, g_coll.level_count + pos,
+          sizeof(uint64_t) * (g_coll.num_levels - pos));
   g_coll.levels[pos] = obj;
+  g_coll.level_count[pos] = 1;
   g_coll.num_levels++;
 }
 

@@ -121,6 +121,9 @@ SIGNATURES = {
     "sqsg_result_get": (C.c_int, [C.c_void_p, C.c_uint64, f64p, u8p, f64p, f64p, C.c_char_p,
                                   C.c_size_t, u64p]),
     "sqsg_result_species_packed": (C.c_int, [C.c_void_p, C.c_uint64, u8p]),
+    "sqsg_result_levels": (C.c_int, [C.c_void_p, f64p, u64p]),
+    "sqsg_result_bulk": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, f64p, u64p, u8p, u64p]),
+    "sqsg_result_sro_bulk": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, f64p]),
     "sqsg_batch_run": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int,
                                  C.POINTER(BatchReport)]),
     "sqsg_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), f64p]),

@@ -734,35 +734,62 @@ class SqsCallbackContext:
 
 
 class SqsResult:
-    """One reported configuration (core/results.h:142-228 for interact mode)."""
+    """One reported configuration (core/results.h:142-283).  Species are in the original
+    site order as atomic numbers; SRO and rank are fetched from the job on first use."""
 
-    def __init__(self, pack: "SqsResultPack", objective: float, species: list[int],
-                 sub_objectives: np.ndarray, sro: list[np.ndarray], rank: str, packed: np.ndarray,
-                 candidate: int):
+    def __init__(self, pack: "SqsResultPack", index: int, objective: float, species):
         self._pack = pack
+        self.index = index
         self.objective = objective
-        self.species = species
-        self._sub = sub_objectives
-        self._sro = sro
-        self._rank = rank
-        self.packed = packed
-        self.candidate = candidate
+        self.species = [int(z) for z in species]
+        self._sro = None
+        self._sub = None
+        self._rank = None
+        self._cand = None
+
+    def _fetch(self):
+        job = self._pack._job
+        nprob = len(self._pack._problems)
+        sub = np.empty(nprob)
+        sro = np.empty(self._pack._cube_total)
+        rank = C.create_string_buffer(1 << 16)
+        cand = C.c_uint64()
+        L.check(L.lib().sqsg_result_get(job.handle, self.index, None, None, L.ptr(sub, L.f64p),
+                                        L.ptr(sro, L.f64p), rank, len(rank), C.byref(cand)))
+        cubes, off = [], 0
+        for p in self._pack._problems:
+            n = p["num_shells"] * p["num_species"] ** 2
+            cubes.append(sro[off:off + n].reshape(p["num_shells"], p["num_species"],
+                                                  p["num_species"]))
+            off += n
+        self._sro, self._sub, self._rank, self._cand = cubes, sub, rank.value.decode(), cand.value
+
+    @property
+    def candidate(self) -> int:
+        if self._cand is None:
+            self._fetch()
+        return self._cand
 
     def sro(self, *args):
         if self._pack.config.sublattice_mode == split:
             raise ValueError("split-mode results expose SRO per sublattice (.sublattices)")
+        if self._sro is None:
+            self._fetch()
         return self._sro[0].copy() if not args else self.parameter(*args)
 
     def parameter(self, shell: int, i=None, j=None):
+        if self._sro is None:
+            self._fetch()
         info = self._pack._problems[0]
         s = info["shell_ids"].index(shell)
         if i is None:
             return self._sro[0][s].copy()
-        zi, zj = to_z(i), to_z(j)
-        a, b = info["species_z"].index(zi), info["species_z"].index(zj)
+        a, b = info["species_z"].index(to_z(i)), info["species_z"].index(to_z(j))
         return float(self._sro[0][s, a, b])
 
     def rank(self) -> str:
+        if self._rank is None:
+            self._fetch()
         return self._rank
 
     def structure(self) -> Structure:
@@ -770,12 +797,12 @@ class SqsResult:
 
     @property
     def sublattices(self) -> list["SqsSublatticeResult"]:
-        out = []
+        if self._sro is None:
+            self._fetch()
         ranks = self._rank.split(",")
-        for q, info in enumerate(self._pack._problems):
-            out.append(SqsSublatticeResult(float(self._sub[q]), self._sro[q], ranks[q]
-                                           if q < len(ranks) else "", info))
-        return out
+        return [SqsSublatticeResult(float(self._sub[q]), self._sro[q],
+                                    ranks[q] if q < len(ranks) else "", info)
+                for q, info in enumerate(self._pack._problems)]
 
 
 @dataclass
@@ -793,61 +820,69 @@ class SqsSublatticeResult:
 
 
 class SqsResultPack:
-    """core/results.h:336-389: (objective, [results]) in ascending objective order."""
+    """core/results.h:336-389: (objective, [results]) in ascending objective order.
 
-    def __init__(self):
-        self.config: Optional[SqsConfiguration] = None
-        self.statistics: dict = {}
-        self._levels: list[tuple[float, list[SqsResult]]] = []
-        self._problems: list[dict] = []
+    The results stay in the job (host memory of the C++ layer); levels can hold millions of
+    ties, so per-result objects are built only when a level is iterated, and the bulk
+    accessors return numpy arrays."""
 
-    @classmethod
-    def _from_job(cls, job: Job) -> "SqsResultPack":
-        pack = cls()
-        pack.config = job.config
-        pack.statistics = job.statistics()
-        pack._problems = [job.problem(q) for q in range(job.num_problems())]
+    def __init__(self, job: "Job"):
+        self._job = job
+        self.config = job.config
+        self.statistics = job.statistics()
+        self._problems = [job.problem(q) for q in range(job.num_problems())]
+        self._cube_total = sum(p["num_shells"] * p["num_species"] ** 2 for p in self._problems)
         nr, nl = C.c_uint64(), C.c_uint64()
         L.check(L.lib().sqsg_result_count(job.handle, C.byref(nr), C.byref(nl)))
-        N = len(job.config.structure())
-        nprob = len(pack._problems)
-        cubes = [p["num_shells"] * p["num_species"] ** 2 for p in pack._problems]
-        blob = sum(p["num_sites"] for p in pack._problems)
-        rank = C.create_string_buffer(1 << 16)
-        levels: dict[float, list[SqsResult]] = {}
-        for r in range(nr.value):
-            obj = C.c_double()
-            sz = np.empty(N, dtype=np.uint8)
-            sub = np.empty(nprob)
-            sro = np.empty(sum(cubes))
-            cand = C.c_uint64()
-            packed = np.empty(blob, dtype=np.uint8)
-            L.check(L.lib().sqsg_result_get(job.handle, r, C.byref(obj), L.ptr(sz, L.u8p),
-                                            L.ptr(sub, L.f64p), L.ptr(sro, L.f64p), rank,
-                                            len(rank), C.byref(cand)))
-            L.check(L.lib().sqsg_result_species_packed(job.handle, r, L.ptr(packed, L.u8p)))
-            cubes_r, off = [], 0
-            for p, c in zip(pack._problems, cubes):
-                cubes_r.append(sro[off:off + c].reshape(p["num_shells"], p["num_species"],
-                                                        p["num_species"]))
-                off += c
-            res = SqsResult(pack, obj.value, sz.tolist(), sub, cubes_r, rank.value.decode(),
-                            packed, cand.value)
-            levels.setdefault(obj.value, []).append(res)
-        pack._levels = sorted(levels.items(), key=lambda kv: kv[0])
-        return pack
+        self._n = nr.value
+        self.level_objectives = np.empty(nl.value)
+        self.level_start = np.empty(nl.value + 1, dtype=np.uint64)
+        L.check(L.lib().sqsg_result_levels(job.handle, L.ptr(self.level_objectives, L.f64p),
+                                           L.ptr(self.level_start, L.u64p)))
+
+    @classmethod
+    def _from_job(cls, job: "Job") -> "SqsResultPack":
+        return cls(job)
+
+    def arrays(self, first: int = 0, count: Optional[int] = None, species: bool = True,
+               ranks: bool = True) -> dict:
+        """objective, candidate, original-order species (Z) and uint64 rank arrays"""
+        count = self._n - first if count is None else count
+        N = len(self.config.structure())
+        obj = np.empty(count)
+        cand = np.empty(count, dtype=np.uint64)
+        sp = np.empty((count, N), dtype=np.uint8) if species else None
+        rk = np.empty(count, dtype=np.uint64) if ranks else None
+        L.check(L.lib().sqsg_result_bulk(self._job.handle, first, count, L.ptr(obj, L.f64p),
+                                         L.ptr(cand, L.u64p),
+                                         L.ptr(sp, L.u8p) if species else None,
+                                         L.ptr(rk, L.u64p) if ranks else None))
+        return dict(objective=obj, candidate=cand, species=sp, rank=rk)
+
+    def sro_array(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        count = self._n - first if count is None else count
+        out = np.empty((count, self._cube_total))
+        L.check(L.lib().sqsg_result_sro_bulk(self._job.handle, first, count, L.ptr(out, L.f64p)))
+        return out
+
+    def _level(self, l: int) -> list[SqsResult]:
+        a, b = int(self.level_start[l]), int(self.level_start[l + 1])
+        arr = self.arrays(a, b - a, species=True, ranks=False)
+        return [SqsResult(self, a + k, float(arr["objective"][k]), arr["species"][k])
+                for k in range(b - a)]
 
     def __iter__(self):
-        return iter(self._levels)
+        for l in range(len(self.level_objectives)):
+            yield float(self.level_objectives[l]), self._level(l)
 
     def __len__(self) -> int:
-        return len(self._levels)
+        return len(self.level_objectives)
 
     def num_results(self) -> int:
-        return sum(len(v) for _, v in self._levels)
+        return self._n
 
     def best(self) -> SqsResult:
-        return self._levels[0][1][0]
+        return self._level(0)[0]
 
 
 def optimize(config: Union[dict, SqsConfiguration, str], level: Any = None,
@@ -857,8 +892,5 @@ def optimize(config: Union[dict, SqsConfiguration, str], level: Any = None,
     if isinstance(cfg, ParseError):
         raise ValueError(f"{cfg.key}: {cfg.msg}")
     job = Job(cfg)
-    try:
-        job.run(callback)
-        return job.results()
-    finally:
-        job.close()
+    job.run(callback)
+    return job.results()

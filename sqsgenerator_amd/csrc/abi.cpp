@@ -305,8 +305,8 @@ int sqsg_job_timing(sqsg_job *job, double *kernel_ms, uint64_t *launches, uint64
 int sqsg_result_count(sqsg_job *job, uint64_t *num_results, uint64_t *num_objectives) {
   return guard([&] {
     need(job, "job");
-    if (num_results) *num_results = job->results.size();
-    if (num_objectives) *num_objectives = job->num_levels;
+    if (num_results) *num_results = job->res.size();
+    if (num_objectives) *num_objectives = job->res.level_obj.size();
   });
 }
 
@@ -315,22 +315,50 @@ int sqsg_result_get(sqsg_job *job, uint64_t r, double *objective, uint8_t *speci
                     uint64_t *candidate) {
   return guard([&] {
     need(job, "job");
-    if (r >= job->results.size()) throw std::out_of_range("result index out of range");
-    const auto &res = job->results[r];
-    if (objective) *objective = res.objective;
-    if (species_z) std::memcpy(species_z, res.species_z.data(), res.species_z.size());
-    if (sub_objectives) std::memcpy(sub_objectives, res.sub.data(), res.sub.size() * sizeof(double));
-    if (sro) std::memcpy(sro, res.sro.data(), res.sro.size() * sizeof(double));
-    if (rank) copy_str(res.rank, rank, rank_cap);
-    if (candidate) *candidate = res.cand;
+    const auto &R = job->res;
+    if (r >= R.size()) throw std::out_of_range("result index out of range");
+    const size_t nprob = job->problems.size();
+    if (objective) *objective = sqsg::result_objective(job, r);
+    if (species_z) sqsg::result_species_z(job, r, species_z);
+    if (sub_objectives)
+      std::memcpy(sub_objectives, R.sub.data() + r * nprob, nprob * sizeof(double));
+    if (sro) sqsg::result_sro(job, r, 1, sro);
+    if (rank) copy_str(sqsg::result_rank(job, r), rank, rank_cap);
+    if (candidate) *candidate = R.cand[r];
   });
 }
 
 int sqsg_result_species_packed(sqsg_job *job, uint64_t r, uint8_t *species) {
   return guard([&] {
     need(job, "job"), need(species, "species");
-    if (r >= job->results.size()) throw std::out_of_range("result index out of range");
-    std::memcpy(species, job->results[r].packed.data(), job->results[r].packed.size());
+    if (r >= job->res.size()) throw std::out_of_range("result index out of range");
+    std::memcpy(species, job->res.species.data() + r * job->stride, job->stride);
+  });
+}
+
+int sqsg_result_levels(sqsg_job *job, double *objectives, uint64_t *level_start) {
+  return guard([&] {
+    need(job, "job");
+    const auto &R = job->res;
+    if (objectives)
+      std::memcpy(objectives, R.level_obj.data(), R.level_obj.size() * sizeof(double));
+    if (level_start)
+      std::memcpy(level_start, R.level_start.data(), R.level_start.size() * sizeof(uint64_t));
+  });
+}
+
+int sqsg_result_bulk(sqsg_job *job, uint64_t first, uint64_t count, double *objective,
+                     uint64_t *candidate, uint8_t *species_z, uint64_t *rank) {
+  return guard([&] {
+    need(job, "job");
+    sqsg::result_bulk(job, first, count, objective, candidate, species_z, rank);
+  });
+}
+
+int sqsg_result_sro_bulk(sqsg_job *job, uint64_t first, uint64_t count, double *sro) {
+  return guard([&] {
+    need(job, "job"), need(sro, "sro");
+    sqsg::result_sro(job, first, count, sro);
   });
 }
 

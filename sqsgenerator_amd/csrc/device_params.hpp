@@ -23,6 +23,12 @@ struct DevProblem {
   const uint32_t *nbr_start;  // N + 1 CSR offsets of the upper neighbour lists
   const uint32_t *nbr;        // P entries j | shell << 16 (per site i, j > i)
   const int8_t *adj;          // dense: [S][Npad][Npad] upper-triangular 0/1 (transposed: [s][j][i])
+  // lane-per-candidate layout: per shell s, groups [i | count << 16] followed by `count`
+  // upper neighbours j of site i; lane_off[s] .. lane_off[s + 1] delimit shell s
+  const uint32_t *lane_stream;
+  const uint32_t *lane_off;
+  const uint8_t *species0F;   // species0 * F (field shift of each species), 4-byte padded
+  uint32_t F;                 // bits per packed neighbour-count field (K * F <= 32)
   uint32_t N, K, S, P, T;
   uint32_t Npad;              // N rounded up to the dense tile (32)
   uint32_t nbounds;

@@ -178,9 +178,48 @@ void Problem::upload_device() {
   upload(d_terms, terms);
 }
 
+void Problem::build_lane(uint32_t field_bits) {
+  F = field_bits;
+  const uint32_t cap = (1u << F) - 1u;
+  // upper neighbours per (shell, site), ascending j, split into groups of at most `cap`
+  std::vector<std::vector<uint32_t>> nb(size_t(S) * N);
+  for (size_t q = 0; q < pi.size(); ++q) nb[size_t(ps[q]) * N + pi[q]].push_back(pj[q]);
+  lane_stream.clear();
+  lane_off.assign(S + 1, 0);
+  for (uint32_t s = 0; s < S; ++s) {
+    lane_off[s] = static_cast<uint32_t>(lane_stream.size());
+    for (uint32_t i = 0; i < N; ++i) {
+      const auto &js = nb[size_t(s) * N + i];
+      for (size_t k = 0; k < js.size(); k += cap) {
+        const uint32_t cnt = static_cast<uint32_t>(std::min<size_t>(cap, js.size() - k));
+        lane_stream.push_back(i | (cnt << 16));
+        lane_stream.insert(lane_stream.end(), js.begin() + k, js.begin() + k + cnt);
+      }
+    }
+  }
+  lane_off[S] = static_cast<uint32_t>(lane_stream.size());
+  std::vector<uint8_t> sF(species_packed.size());
+  for (size_t i = 0; i < sF.size(); ++i) sF[i] = static_cast<uint8_t>(species_packed[i] * F);
+  upload(d_lane_stream, lane_stream);
+  upload(d_lane_off, lane_off);
+  upload(d_species0F, sF, 16);
+}
+
 void ensure_device(sqsg_job *job) {
   if (job->on_device) return;
   for (auto &P : job->problems) P.upload_device();
+  job->kmax = 0;
+  for (const auto &P : job->problems) job->kmax = std::max(job->kmax, P.K);
+  job->lane_ok = job->kmax <= 8;
+  for (const auto &P : job->problems) {
+    if (P.N >= 65536) job->lane_ok = false;
+    std::vector<size_t> per_shell(P.S, 0);
+    for (uint32_t s : P.ps) ++per_shell[s];
+    for (size_t c : per_shell)
+      if (c >= 65536) job->lane_ok = false;  // 16-bit histogram fields
+  }
+  if (job->lane_ok)
+    for (auto &P : job->problems) P.build_lane(lane_field_bits(job->kmax));
   SQSG_HIP(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking));
   SQSG_HIP(hipEventCreate(&job->ev0));
   SQSG_HIP(hipEventCreate(&job->ev1));
@@ -195,6 +234,10 @@ DevProblem Problem::dev() const {
   d.pw = d_pw.as<const double>();
   d.tgt = d_tgt.as<const double>();
   d.terms = d_terms.as<const uint32_t>();
+  d.lane_stream = d_lane_stream.as<const uint32_t>();
+  d.lane_off = d_lane_off.as<const uint32_t>();
+  d.species0F = d_species0F.as<const uint8_t>();
+  d.F = F;
   d.nbr_start = d_nbr_start.as<const uint32_t>();
   d.nbr = d_nbr.as<const uint32_t>();
   d.adj = d_adj.as<const int8_t>();
@@ -321,6 +364,24 @@ sqsg_job *create_job(const sqsg_config &c) {
       build_problem(job->problems[q], in, lat, c, q, false);
     }
   }
+  job->stride = 0;
+  job->cube_total = 0;
+  job->rank_fast = true;
+  for (const auto &P : job->problems) {
+    job->stride += P.N;
+    job->cube_total += size_t(P.S) * P.K * P.K;
+  }
+  // reported ranks (results.h:163-169) fit 128-bit arithmetic when multinomial * n < 2^128
+  auto fits = [](const BigUint &m, size_t n) {
+    BigUint t(m);
+    t.mul_small(static_cast<uint32_t>(n));
+    return t.fits_u128();
+  };
+  for (const auto &P : job->problems) {
+    std::vector<uint8_t> orig(P.N);
+    for (uint32_t k = 0; k < P.N; ++k) orig[k] = P.species_packed[k];
+    if (!fits(num_permutations(orig.data(), orig.size()), orig.size())) job->rank_fast = false;
+  }
   if (systematic) {
     const Problem &P = job->problems[0];
     const BigUint total =
@@ -334,7 +395,23 @@ sqsg_job *create_job(const sqsg_config &c) {
   return job.release();
 }
 
-RunParams make_run_params(sqsg_job *job, uint32_t W, uint32_t *blocks) {
+int resolve_kernel(const sqsg_job *job, int requested) {
+  switch (requested) {
+    case SQSG_KERNEL_AUTO:
+      return job->lane_ok ? SQSG_KERNEL_SPARSE_LANE : SQSG_KERNEL_SPARSE_WAVE;
+    case SQSG_KERNEL_SPARSE_WAVE:
+      return requested;
+    case SQSG_KERNEL_SPARSE_LANE:
+      if (!job->lane_ok)
+        throw std::invalid_argument("the lane kernel needs at most 8 species and fewer than "
+                                    "65536 pairs per shell");
+      return requested;
+    default:
+      throw std::invalid_argument("kernel variant not available for this job");
+  }
+}
+
+RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *blocks) {
   RunParams p{};
   p.nprob = static_cast<uint32_t>(job->problems.size());
   uint32_t stride_sum = 0, max_bins = 0, max_terms = 0;
@@ -349,123 +426,44 @@ RunParams make_run_params(sqsg_job *job, uint32_t W, uint32_t *blocks) {
   p.chunk_quirk = (!p.systematic && job->sublattice_mode == SQSG_SUBLATTICE_INTERACT) ? 1u : 0u;
   p.chunk_size = job->chunk_size;
   p.L = static_cast<uint32_t>(job->keep + 1);
-  uint32_t G = 64;
-  while (G > 8 && static_cast<size_t>(G) * stride_sum > 24576) G >>= 1;
-  p.gen_group = G;
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   uint32_t off = 0;
-  for (uint32_t q = 0; q < p.nprob; ++q) {
-    p.off_rows[q] = off;
-    off = align(off + G * job->problems[q].row_stride());
+  if (variant == SQSG_KERNEL_SPARSE_LANE) {
+    p.gen_group = 64;
+    for (uint32_t q = 0; q < p.nprob; ++q) {
+      p.off_rows[q] = off;
+      off = align(off + 64 * job->problems[q].N);
+    }
+    p.off_hist = off;
+    off = align(off + job->kmax * lane_field_bits(job->kmax) * 64 * 4);
+  } else {
+    uint32_t G = 64;
+    while (G > 8 && static_cast<size_t>(G) * stride_sum > 24576) G >>= 1;
+    p.gen_group = G;
+    for (uint32_t q = 0; q < p.nprob; ++q) {
+      p.off_rows[q] = off;
+      off = align(off + G * job->problems[q].row_stride());
+    }
+    p.off_hist = off;
+    off = align(off + 4 * max_bins);
+    p.off_terms = off;
+    off = align(off + 8 * max_terms);
+    p.off_obj = off;
+    off = align(off + 8 * 64);
   }
-  p.off_hist = off;
-  off = align(off + 4 * max_bins);
-  p.off_terms = off;
-  off = align(off + 8 * max_terms);
-  p.off_obj = off;
-  off = align(off + 8 * 64);
   p.wave_lds = off;
   const size_t block_lds = static_cast<size_t>(W) * p.wave_lds;
   if (block_lds > 160 * 1024) throw std::out_of_range("problem does not fit the LDS budget");
   uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / block_lds);
-  per_cu = std::max(1u, std::min(per_cu, 8u / std::max(1u, W / 2)));
+  per_cu = std::max(1u, std::min(per_cu, 16u / W));
   *blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
   return p;
 }
 
-void merge_records(sqsg_job *job, std::vector<Record> &&fresh) {
-  auto &rec = job->records;
-  for (auto &r : fresh) rec.push_back(std::move(r));
-  std::vector<double> objs;
-  objs.reserve(rec.size());
-  for (const auto &r : rec) objs.push_back(r.obj);
-  std::sort(objs.begin(), objs.end());
-  objs.erase(std::unique(objs.begin(), objs.end()), objs.end());
-  const size_t L = job->keep + 1;
-  const double v = objs.size() >= L ? objs[L - 1] : std::numeric_limits<double>::infinity();
-  job->vkeep = v;
-  rec.erase(std::remove_if(rec.begin(), rec.end(), [v](const Record &r) { return r.obj > v; }),
-            rec.end());
-}
-
-void finalize_results(sqsg_job *job) {
-  auto &rec = job->records;
-  std::sort(rec.begin(), rec.end(), [](const Record &a, const Record &b) {
-    return a.obj < b.obj || (a.obj == b.obj && a.cand < b.cand);
-  });
-  job->results.clear();
-  job->num_levels = 0;
-  const uint32_t nprob = static_cast<uint32_t>(job->problems.size());
-  for (size_t a = 0; a < rec.size();) {
-    size_t b = a;
-    while (b < rec.size() && rec[b].obj == rec[a].obj) ++b;
-    // one set per objective keyed on the configuration (results.h:46-53); with
-    // max_results_per_objective the reference inserts while count <= max (sqs.h:423-426)
-    std::set<std::vector<uint8_t>> seen;
-    for (size_t k = a; k < b; ++k) {
-      if (job->max_results >= 0 && seen.size() > static_cast<size_t>(job->max_results)) break;
-      if (!seen.insert(rec[k].species).second) continue;
-      Result r;
-      r.objective = rec[k].obj;
-      r.sub.assign(rec[k].sub, rec[k].sub + nprob);
-      r.packed = rec[k].species;
-      r.cand = rec[k].cand;
-      job->results.push_back(std::move(r));
-    }
-    ++job->num_levels;
-    a = b;
-  }
-  if (job->results.empty()) return;
-  // SRO cubes of the reported configurations, scored on the device
-  const size_t R = job->results.size();
-  size_t off = 0;
-  for (uint32_t q = 0; q < nprob; ++q) {
-    const Problem &P = job->problems[q];
-    const size_t nb = size_t(P.S) * P.K * P.K;
-    std::vector<uint8_t> sp(R * P.N);
-    for (size_t r = 0; r < R; ++r)
-      std::memcpy(&sp[r * P.N], job->results[r].packed.data() + off, P.N);
-    DevBuf d_sp(sp.size()), d_obj(R * sizeof(double)), d_sro(R * nb * sizeof(double));
-    SQSG_HIP(hipMemcpy(d_sp.as<void>(), sp.data(), sp.size(), hipMemcpyHostToDevice));
-    score_batch(job, q, d_sp.as<uint8_t>(), static_cast<uint32_t>(R), nullptr,
-                d_obj.as<double>(), d_sro.as<double>(), SQSG_KERNEL_SPARSE_WAVE, job->stream);
-    std::vector<double> obj(R), sro(R * nb);
-    SQSG_HIP(hipMemcpyAsync(obj.data(), d_obj.as<void>(), R * sizeof(double),
-                            hipMemcpyDeviceToHost, job->stream));
-    SQSG_HIP(hipMemcpyAsync(sro.data(), d_sro.as<void>(), sro.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, job->stream));
-    SQSG_HIP(hipStreamSynchronize(job->stream));
-    for (size_t r = 0; r < R; ++r) {
-      if (obj[r] != job->results[r].sub[q])
-        throw DeviceError("re-scored objective differs from the recorded one");
-      job->results[r].sro.insert(job->results[r].sro.end(), sro.begin() + r * nb,
-                                 sro.begin() + (r + 1) * nb);
-    }
-    off += P.N;
-  }
-  // original site order and atomic numbers (sqs.h:94-122), reported ranks (results.h:163-169)
-  for (auto &r : job->results) {
-    r.species_z = job->structure_z;
-    if (job->sublattice_mode == SQSG_SUBLATTICE_INTERACT) {
-      const Problem &P = job->problems[0];
-      for (uint32_t k = 0; k < P.N; ++k) r.species_z[P.sort_order[k]] = P.zmap[r.packed[k]];
-      uint8_t zidx[256] = {0};
-      for (uint32_t k = 0; k < P.K; ++k) zidx[P.zmap[k]] = static_cast<uint8_t>(k);
-      std::vector<uint8_t> orig(P.N);
-      for (uint32_t i = 0; i < P.N; ++i) orig[i] = zidx[r.species_z[i]];
-      r.rank = rank_permutation(orig.data(), orig.size()).str();
-    } else {
-      size_t o = 0;
-      std::string ranks;
-      for (const auto &P : job->problems) {
-        for (uint32_t k = 0; k < P.N; ++k) r.species_z[P.sites[k]] = P.zmap[r.packed[o + k]];
-        if (!ranks.empty()) ranks += ",";
-        ranks += rank_permutation(r.packed.data() + o, P.N).str();
-        o += P.N;
-      }
-      r.rank = ranks;
-    }
-  }
+hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, uint32_t blocks,
+                          uint32_t W, hipStream_t s) {
+  if (variant == SQSG_KERNEL_SPARSE_LANE) return launch_run_lane(p, job->kmax, blocks, W, s);
+  return launch_run_wave(p, blocks, W, s);
 }
 
 namespace {
@@ -497,17 +495,16 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   const auto t_start = clock::now();
   ensure_device(job);
   job->cancel.store(false);
-  job->records.clear();
-  job->results.clear();
-  job->vkeep = std::numeric_limits<double>::infinity();
+  reset_results(job);
   job->stats = sqsg_statistics{};
   job->stats.best_objective = std::numeric_limits<double>::infinity();
   job->kernel_ms = 0;
   job->launches = 0;
   job->scored = 0;
-  const uint32_t W = 4;
+  const int variant = resolve_kernel(job, job->kernel);
+  const uint32_t W = variant == SQSG_KERNEL_SPARSE_LANE ? 2 : 4;
   uint32_t blocks = 0;
-  RunParams p = make_run_params(job, W, &blocks);
+  RunParams p = make_run_params(job, variant, W, &blocks);
   const uint64_t launch = job->launch_size ? job->launch_size : default_launch_size(job);
   if (p.systematic) {
     const uint64_t lanes = uint64_t(blocks) * W * p.gen_group;
@@ -515,8 +512,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   } else {
     p.rounds = 1;
   }
-  uint32_t stride = 0;
-  for (const auto &P : job->problems) stride += P.N;
+  const uint32_t stride = job->stride;
   p.rec_stride = stride;
   RecordSink sink;
   sink.ensure(static_cast<uint32_t>(std::min<uint64_t>(launch, 1u << 16)), stride, p.nprob);
@@ -544,7 +540,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     SQSG_HIP(hipMemsetAsync(d_count.as<void>(), 0, sizeof(unsigned int), job->stream));
     SQSG_HIP(hipMemsetAsync(d_scored.as<void>(), 0, sizeof(unsigned long long), job->stream));
     SQSG_HIP(hipEventRecord(job->ev0, job->stream));
-    SQSG_HIP(launch_run_wave(p, blocks, W, job->stream));
+    SQSG_HIP(launch_variant(p, job, variant, blocks, W, job->stream));
     SQSG_HIP(hipEventRecord(job->ev1, job->stream));
     unsigned int count = 0;
     unsigned long long scored = 0;
@@ -563,7 +559,6 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     }
     if (scored != end - begin) throw DeviceError("device scored an unexpected candidate count");
     job->scored += scored;
-    std::vector<Record> fresh(count);
     if (count) {
       std::vector<uint64_t> cand(count);
       std::vector<double> obj(count), sub(size_t(count) * p.nprob);
@@ -575,27 +570,20 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
       SQSG_HIP(hipMemcpy(sub.data(), sink.sub.as<void>(), sub.size() * sizeof(double),
                          hipMemcpyDeviceToHost));
       SQSG_HIP(hipMemcpy(sp.data(), sink.species.as<void>(), sp.size(), hipMemcpyDeviceToHost));
-      for (uint32_t k = 0; k < count; ++k) {
-        fresh[k].cand = cand[k];
-        fresh[k].obj = obj[k];
-        for (uint32_t q = 0; q < p.nprob; ++q) fresh[k].sub[q] = sub[size_t(k) * p.nprob + q];
-        fresh[k].species.assign(sp.begin() + size_t(k) * stride,
-                                sp.begin() + size_t(k + 1) * stride);
-      }
+      merge_records(job, cand.data(), obj.data(), sub.data(), sp.data(), count);
+      for (uint32_t k = 0; k < count; ++k)
+        if (obj[k] < job->stats.best_objective ||
+            (obj[k] == job->stats.best_objective && cand[k] < job->stats.best_rank)) {
+          job->stats.best_objective = obj[k];
+          job->stats.best_rank = cand[k];
+        }
     }
-    merge_records(job, std::move(fresh));
     // keep the device threshold at the tighter of its own bound and the exact host one
     const unsigned long long hk = objective_key(job->vkeep);
     unsigned long long dk = 0;
     SQSG_HIP(hipMemcpy(&dk, d_thr.as<void>(), sizeof(dk), hipMemcpyDeviceToHost));
     if (hk < dk) SQSG_HIP(hipMemcpy(d_thr.as<void>(), &hk, sizeof(hk), hipMemcpyHostToDevice));
     job->stats.finished = job->scored;
-    for (const auto &r : job->records)
-      if (r.obj < job->stats.best_objective ||
-          (r.obj == job->stats.best_objective && r.cand < job->stats.best_rank)) {
-        job->stats.best_objective = r.obj;
-        job->stats.best_rank = r.cand;
-      }
     begin = end;
     if (cb) {
       const auto now = clock::now();
@@ -616,8 +604,8 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
                  int32_t *d_bonds, double *d_obj, double *d_sro, int kernel, hipStream_t s) {
   if (problem >= job->problems.size()) throw std::out_of_range("problem index out of range");
   ensure_device(job);
-  if (kernel != SQSG_KERNEL_AUTO && kernel != SQSG_KERNEL_SPARSE_WAVE)
-    throw std::invalid_argument("kernel variant not available for explicit scoring");
+  const int variant = kernel == SQSG_KERNEL_AUTO ? SQSG_KERNEL_SPARSE_WAVE
+                                                 : resolve_kernel(job, kernel);
   const Problem &P = job->problems[problem];
   ScoreParams sp{};
   sp.prob = P.dev();
@@ -627,6 +615,16 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
   sp.objective = d_obj;
   sp.sro = d_sro;
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
+  if (variant == SQSG_KERNEL_SPARSE_LANE) {
+    sp.off_row = 0;
+    sp.off_hist = align(64 * P.N);
+    sp.wave_lds = align(sp.off_hist + job->kmax * lane_field_bits(job->kmax) * 256);
+    if (sp.wave_lds > 160 * 1024) throw std::out_of_range("problem exceeds LDS");
+    SQSG_HIP(launch_score_lane(sp, job->kmax, s ? s : job->stream));
+    return;
+  }
+  if (variant != SQSG_KERNEL_SPARSE_WAVE)
+    throw std::invalid_argument("kernel variant not available for explicit scoring");
   sp.off_row = 0;
   sp.off_hist = align(P.row_stride());
   sp.off_terms = align(sp.off_hist + 4 * P.S * P.K * P.K);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <atomic>
+#include <map>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -73,27 +74,34 @@ struct Problem {
   uint8_t sys_rev[kMaxLocalSpecies] = {};
   // device copies
   DevBuf d_pairs, d_species0, d_pref, d_pw, d_tgt, d_terms, d_nbr_start, d_nbr, d_adj;
+  // lane-per-candidate layout (built for the job's field width F)
+  std::vector<uint32_t> lane_stream, lane_off;
+  uint32_t F = 0;
+  DevBuf d_lane_stream, d_lane_off, d_species0F;
+  void build_lane(uint32_t field_bits);
   uint32_t Npad = 0;
   DevProblem dev() const;
   void upload_device();
   uint32_t row_stride() const { return (((N + 3) / 4) | 1u) * 4u; }
 };
 
-struct Record {
-  uint64_t cand;
-  double obj;
-  double sub[kMaxProblems];
-  std::vector<uint8_t> species;  // concatenated per problem, sorted-site packed
+// Records of one objective level gathered from the device record sinks.
+struct LevelRecords {
+  std::vector<uint64_t> cand;
+  std::vector<double> sub;        // one objective per problem and record
+  std::vector<uint8_t> species;   // sorted-site packed species, `stride` bytes per record
 };
 
-struct Result {
-  double objective;
+// Final, deduplicated results in ascending objective order (the sqs_result_pack levels).
+struct ResultSet {
+  std::vector<double> level_obj;
+  std::vector<uint64_t> level_start;  // levels + 1 offsets into the per-result arrays
+  std::vector<uint64_t> cand;
   std::vector<double> sub;
-  std::vector<uint8_t> packed;     // concatenated per problem
-  std::vector<uint8_t> species_z;  // whole structure, original order
-  std::vector<double> sro;         // concatenated [S][K][K] per problem
-  std::string rank;
-  uint64_t cand;
+  std::vector<uint8_t> species;
+  std::vector<double> sro;            // cached SRO cubes (all problems) or empty
+  size_t size() const { return cand.size(); }
+  void clear() { *this = ResultSet(); }
 };
 
 }  // namespace sqsg
@@ -110,15 +118,19 @@ struct sqsg_job {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool on_device = false;
+  uint32_t kmax = 0;      // largest species count over the problems
+  bool lane_ok = false;   // the lane-per-candidate kernel applies
   std::atomic<bool> cancel{false};
   // run state
-  std::vector<sqsg::Record> records;
+  std::map<double, sqsg::LevelRecords> levels;  // levels at or below the keep threshold
   double vkeep = 0;
+  uint32_t stride = 0;        // packed species bytes per record (sum of problem sites)
+  size_t cube_total = 0;      // doubles of one result's SRO cubes
+  bool rank_fast = false;     // reported ranks fit 128-bit arithmetic
   sqsg_statistics stats{};
   double kernel_ms = 0;
   uint64_t launches = 0, scored = 0;
-  std::vector<sqsg::Result> results;
-  uint64_t num_levels = 0;
+  sqsg::ResultSet res;
   ~sqsg_job();
 };
 
@@ -129,9 +141,24 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user);
 void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint32_t B,
                  int32_t *d_bonds, double *d_obj, double *d_sro, int kernel, hipStream_t s);
 // helpers shared with the batched and serving drivers
-RunParams make_run_params(sqsg_job *job, uint32_t waves_per_block, uint32_t *blocks);
-void merge_records(sqsg_job *job, std::vector<Record> &&fresh);
+RunParams make_run_params(sqsg_job *job, int variant, uint32_t waves_per_block,
+                          uint32_t *blocks);
+// kernel variant a run uses (SQSG_KERNEL_SPARSE_WAVE / _LANE / _DENSE_MFMA)
+int resolve_kernel(const sqsg_job *job, int requested);
+hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, uint32_t blocks,
+                          uint32_t waves_per_block, hipStream_t s);
+void reset_results(sqsg_job *job);
+void merge_records(sqsg_job *job, const uint64_t *cand, const double *obj, const double *sub,
+                   const uint8_t *species, uint32_t count);
 void finalize_results(sqsg_job *job);
+// accessors of finalized result r (original site order, atomic numbers)
+void result_species_z(const sqsg_job *job, uint64_t r, uint8_t *out);
+std::string result_rank(const sqsg_job *job, uint64_t r);
+bool result_rank_u64(const sqsg_job *job, uint64_t r, uint64_t *rank);
+void result_sro(sqsg_job *job, uint64_t first, uint64_t count, double *out);
+double result_objective(const sqsg_job *job, uint64_t r);
+void result_bulk(sqsg_job *job, uint64_t first, uint64_t count, double *objective,
+                 uint64_t *candidate, uint8_t *species_z, uint64_t *rank);
 int device_cu_count();
 void ensure_device(sqsg_job *job);
 

@@ -14,34 +14,11 @@
 // sequential, non-fused evaluation because results are keyed on exact objective values.
 #include <hip/hip_runtime.h>
 
+#include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace sqsg {
 namespace {
-
-constexpr uint64_t kSecret0 = 0x2d358dccaa6c78a5ull;  // rapidhash.h:129-131
-constexpr uint64_t kSecret1 = 0x8bb84b93962eacc9ull;
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// rapidrand (shuffle.h:23-26): state += s0; mix(state, state ^ s1) = lo ^ hi of the
-// 128-bit product.  random_bounded (shuffle.h:28-32) only uses the low 32 bits.
-__device__ __forceinline__ uint32_t draw32(uint64_t &st) {
-  st += kSecret0;
-  const uint64_t b = st ^ kSecret1;
-  return static_cast<uint32_t>(st * b) ^ static_cast<uint32_t>(__umul64hi(st, b));
-}
-
-__device__ __forceinline__ double absolute(double a) { return a > 0 ? a : -a; }  // numeric.h:15
-
-__device__ __forceinline__ unsigned long long okey(double d) {
-  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(d));
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
 
 // copy the sorted configuration (4-byte padded) into an LDS row
 __device__ __forceinline__ void load_row(uint8_t *row, const DevProblem &pr) {
@@ -65,15 +42,6 @@ __device__ __forceinline__ void shuffle_row(uint8_t *row, const DevProblem &pr, 
       row[lo + p] = t;
     }
   }
-}
-
-// a * m / d for a result below 2^64 (then the high word of a * m is below d)
-__device__ __forceinline__ uint64_t muldiv(uint64_t a, uint32_t m, uint32_t d) {
-  const uint64_t lo = a * m, hi = __umul64hi(a, static_cast<uint64_t>(m));
-  const uint64_t x1 = (hi << 32) | (lo >> 32);
-  const uint64_t q1 = x1 / d, r1 = x1 % d;
-  const uint64_t x0 = (r1 << 32) | (lo & 0xffffffffull);
-  return (q1 << 32) | (x0 / d);
 }
 
 // permutation.h:52-78 on the window bounds[0..1], local species remapped (shuffle.h:69-88)

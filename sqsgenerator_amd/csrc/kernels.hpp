@@ -15,6 +15,13 @@ hipError_t launch_score_explicit(const ScoreParams &p, hipStream_t stream);
 hipError_t launch_run_wave(const RunParams &p, uint32_t blocks, uint32_t waves_per_block,
                            hipStream_t stream);
 
+// one lane per candidate; kmax = largest species count of the job's problems (<= 8)
+hipError_t launch_run_lane(const RunParams &p, uint32_t kmax, uint32_t blocks,
+                           uint32_t waves_per_block, hipStream_t stream);
+hipError_t launch_score_lane(const ScoreParams &p, uint32_t kmax, hipStream_t stream);
+// bits per packed neighbour-count field for a largest species count kmax
+uint32_t lane_field_bits(uint32_t kmax);
+
 // order-preserving key of a double (unsigned order == numeric order)
 inline unsigned long long objective_key(double d) {
   unsigned long long b;

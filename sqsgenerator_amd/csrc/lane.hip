@@ -1,0 +1,332 @@
+// Lane-per-candidate sparse scoring for gfx950: each of the 64 lanes of a wavefront owns
+// one candidate; the wavefront walks the neighbour structure uniformly (scalar loads), so
+// the pair list costs no per-candidate memory traffic.
+//
+// LDS of one wavefront:
+//   rows  [site][64 lanes] bytes      species * F of every lane's candidate
+//   hist  [a * F + w][64 lanes] u32   16-bit count fields of the current shell
+// For every site i and shell s the lane counts its upper neighbours j > i by species in
+// F-bit fields of one register (acc += 1 << (F * b_j)), then adds the fields to row
+// a = species(i) of the shell histogram.  A field of row a, column b therefore ends up as
+// bonds(s, a, b) of core/optimization.h:74-77 (ordered, a = species[i], i < j).  Shells are
+// processed in order and each shell's objective terms are added as soon as its histogram
+// is complete, which is exactly the summation order of optimization.h:85-100.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace sqsg {
+namespace {
+
+template <int KM> struct LaneShape {
+  static constexpr uint32_t F = (32 / KM) > 8 ? 8u : static_cast<uint32_t>(32 / KM);
+  static constexpr uint32_t W = (KM + 1) / 2;  // 16-bit field pairs per histogram row
+  static constexpr uint32_t MASK = (1u << F) - 1u;
+  static_assert(W <= F, "histogram row must fit its stride");
+};
+
+// column access: element of site i of this lane's candidate
+__device__ __forceinline__ uint8_t &at(uint8_t *col, uint32_t i) { return col[i << 6]; }
+
+__device__ __forceinline__ void load_col(uint8_t *col, const DevProblem &pr) {
+  for (uint32_t i = 0; i < pr.N; ++i) at(col, i) = pr.species0F[i];
+}
+
+__device__ __forceinline__ void shuffle_col(uint8_t *col, const DevProblem &pr, uint64_t cand) {
+  uint64_t st = pr.seed + (cand * static_cast<uint64_t>(pr.draws)) * kSecret0;
+  for (uint32_t b = 0; b < pr.nbounds; ++b) {
+    const uint32_t lo = pr.bounds[2 * b], hi = pr.bounds[2 * b + 1];
+    for (uint32_t i = hi - lo; i > 1; --i) {
+      const uint32_t p = static_cast<uint32_t>((static_cast<uint64_t>(draw32(st)) * i) >> 32);
+      const uint8_t t = at(col, lo + i - 1);
+      at(col, lo + i - 1) = at(col, lo + p);
+      at(col, lo + p) = t;
+    }
+  }
+}
+
+__device__ void unrank_col(uint8_t *col, const DevProblem &pr, uint64_t rank) {
+  uint32_t hist[kMaxLocalSpecies];
+  for (uint32_t j = 0; j < kMaxLocalSpecies; ++j) hist[j] = pr.sys_hist[j];
+  const uint32_t lo = pr.bounds[0], w = pr.bounds[1] - pr.bounds[0];
+  uint64_t total = 1;
+  uint32_t placed = 0;
+  for (uint32_t j = 0; j < pr.sys_nspecies; ++j)
+    for (uint32_t k = 1; k <= hist[j]; ++k) total = muldiv(total, ++placed, k);
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < w; ++i)
+    for (uint32_t j = 0; j < pr.sys_nspecies; ++j) {
+      const uint64_t sc = muldiv(total, hist[j], w - i);
+      if (rank <= sc) {
+        at(col, lo + k++) = static_cast<uint8_t>(pr.sys_rev[j] * pr.F);
+        total = sc;
+        --hist[j];
+        break;
+      }
+      rank -= sc;
+    }
+}
+
+__device__ void next_perm_col(uint8_t *col, uint32_t lo, uint32_t w) {
+  if (w < 2) return;
+  int k = static_cast<int>(w) - 2;
+  while (k >= 0 && at(col, lo + k) >= at(col, lo + k + 1)) --k;
+  if (k < 0) return;
+  int l = static_cast<int>(w) - 1;
+  while (at(col, lo + k) >= at(col, lo + l)) --l;
+  uint8_t t = at(col, lo + k);
+  at(col, lo + k) = at(col, lo + l);
+  at(col, lo + l) = t;
+  for (int a = k + 1, b = static_cast<int>(w) - 1; a < b; ++a, --b) {
+    t = at(col, lo + a);
+    at(col, lo + a) = at(col, lo + b);
+    at(col, lo + b) = t;
+  }
+}
+
+template <int KM>
+__device__ __forceinline__ uint32_t field_count(const uint32_t *hl, uint32_t a, uint32_t b) {
+  constexpr uint32_t F = LaneShape<KM>::F;
+  return (hl[(a * F + (b >> 1)) << 6] >> ((b & 1u) << 4)) & 0xffffu;
+}
+
+// Bond histogram + objective of the lane's candidate (`col`), shell by shell.  `hl` is
+// the lane's histogram column (hist + lane).  Optionally writes ordered bonds and SRO.
+template <int KM>
+__device__ double score_lane(const uint8_t *col, const DevProblem &pr, uint32_t *hl,
+                             int32_t *bonds, double *sro) {
+  using Sh = LaneShape<KM>;
+  constexpr uint32_t F = Sh::F, W = Sh::W, MASK = Sh::MASK;
+  const uint32_t K = pr.K;
+  const uint32_t *st = pr.lane_stream;
+  double obj = 0.0;
+  for (uint32_t s = 0; s < pr.S; ++s) {
+#pragma unroll
+    for (uint32_t a = 0; a < KM; ++a)
+#pragma unroll
+      for (uint32_t w = 0; w < W; ++w)
+        if (a < K) hl[(a * F + w) << 6] = 0u;
+    const uint32_t e1 = pr.lane_off[s + 1];
+    for (uint32_t e = pr.lane_off[s]; e < e1;) {
+      const uint32_t hdr = st[e];
+      const uint32_t i = hdr & 0xffffu, cnt = hdr >> 16;
+      const uint32_t *js = st + e + 1;
+      e += 1 + cnt;
+      const uint32_t a6 = col[i << 6];
+      uint32_t acc = 0;
+      uint32_t k = 0;
+      for (; k + 4 <= cnt; k += 4) {
+        const uint32_t b0 = col[js[k] << 6], b1 = col[js[k + 1] << 6];
+        const uint32_t b2 = col[js[k + 2] << 6], b3 = col[js[k + 3] << 6];
+        acc += ((1u << b0) + (1u << b1)) + ((1u << b2) + (1u << b3));
+      }
+      for (; k < cnt; ++k) acc += 1u << col[js[k] << 6];
+      uint32_t *h = hl + (a6 << 6);
+#pragma unroll
+      for (uint32_t w = 0; w < W; ++w) {
+        uint32_t v = (acc >> (2 * w * F)) & MASK;
+        if (2 * w + 1 < KM) v |= ((acc >> ((2 * w + 1) * F)) & MASK) << 16;
+        atomicAdd(h + (w << 6), v);
+      }
+    }
+    // this shell's objective terms, optimization.h:85-100
+    const double *pref = pr.pref + s * K * K;
+    const double *pw = pr.pw + s * K * K;
+    const double *tgt = pr.tgt + s * K * K;
+    for (uint32_t xi = 0; xi < K; ++xi) {
+      const uint32_t cxx = field_count<KM>(hl, xi, xi);
+      const double sxx = 1.0 - static_cast<double>(cxx) * pref[xi * K + xi];
+      obj += pw[xi * K + xi] * absolute(sxx - tgt[xi * K + xi]);
+      if (sro) sro[(s * K + xi) * K + xi] = sxx;
+      if (bonds) bonds[(s * K + xi) * K + xi] = static_cast<int32_t>(cxx);
+      for (uint32_t eta = xi + 1; eta < K; ++eta) {
+        const uint32_t cab = field_count<KM>(hl, xi, eta), cba = field_count<KM>(hl, eta, xi);
+        const double v = 1.0 - static_cast<double>(cab + cba) * pref[xi * K + eta];
+        obj += pw[xi * K + eta] * absolute(v - tgt[xi * K + eta]);
+        if (sro) sro[(s * K + xi) * K + eta] = v, sro[(s * K + eta) * K + xi] = v;
+        if (bonds) {
+          bonds[(s * K + xi) * K + eta] = static_cast<int32_t>(cab);
+          bonds[(s * K + eta) * K + xi] = static_cast<int32_t>(cba);
+        }
+      }
+    }
+  }
+  return obj;
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void run_lane_kernel(RunParams p) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  uint8_t *wl = lds + wave * p.wave_lds;
+  uint32_t *hl = reinterpret_cast<uint32_t *>(wl + p.off_hist) + lane;
+  const uint64_t gw = static_cast<uint64_t>(blockIdx.x) * NW + wave;
+  const uint64_t GW = static_cast<uint64_t>(gridDim.x) * NW;
+  const uint64_t per_seg = 64ull * p.rounds;
+  const uint64_t total = p.cand_end - p.cand_begin;
+  const uint64_t nseg = (total + per_seg - 1) / per_seg;
+  double lvl[kMaxLevels];
+  uint32_t nlvl = 0;
+  unsigned long long scored = 0;
+  for (uint64_t seg = gw; seg < nseg; seg += GW) {
+    const uint64_t sbeg = p.cand_begin + seg * per_seg;
+    for (uint32_t r = 0; r < p.rounds; ++r) {
+      const uint64_t cand = p.systematic ? sbeg + static_cast<uint64_t>(lane) * p.rounds + r
+                                         : sbeg + static_cast<uint64_t>(r) * 64 + lane;
+      const bool active = cand < p.cand_end;
+      // every lane keeps a valid configuration in its column (inactive lanes score the
+      // sorted one and discard it), so every LDS address below stays in the wave's region
+      for (uint32_t q = 0; q < p.nprob; ++q) {
+        const DevProblem &pr = p.prob[q];
+        uint8_t *col = wl + p.off_rows[q] + lane;
+        if (p.systematic) {
+          if (r == 0) {
+            load_col(col, pr);
+            if (active) unrank_col(col, pr, cand + 1);
+          } else if (active) {
+            next_perm_col(col, pr.bounds[0], pr.bounds[1] - pr.bounds[0]);
+          }
+        } else {
+          load_col(col, pr);
+          if (active && !(p.chunk_quirk && cand % p.chunk_size == 0)) shuffle_col(col, pr, cand);
+        }
+      }
+      if (__ballot(active) == 0) break;  // the tail of a systematic segment
+      double sub[kMaxProblems];
+      for (uint32_t q = 0; q < p.nprob; ++q)
+        sub[q] = score_lane<KM>(wl + p.off_rows[q] + lane, p.prob[q], hl, nullptr, nullptr);
+      double tot;
+      if (p.nprob == 1) {
+        tot = sub[0];
+      } else {  // helpers::sum folds from T(0) (core/helpers/fold.h:51-53)
+        tot = 0.0;
+        for (uint32_t q = 0; q < p.nprob; ++q) tot += sub[q];
+      }
+      scored += static_cast<unsigned long long>(__popcll(__ballot(active)));
+      const unsigned long long g =
+          __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const bool pass = active && okey(tot) <= g;
+      if (pass) {
+        const uint32_t slot = atomicAdd(p.rec_count, 1u);
+        if (slot < p.rec_cap) {
+          uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
+          uint32_t off = 0;
+          for (uint32_t q = 0; q < p.nprob; ++q) {
+            const DevProblem &pr = p.prob[q];
+            const uint8_t *col = wl + p.off_rows[q] + lane;
+            for (uint32_t i = 0; i < pr.N; ++i) dst[off + i] = col[i << 6] / pr.F;
+            off += pr.N;
+          }
+          p.rec_cand[slot] = cand;
+          p.rec_obj[slot] = tot;
+          for (uint32_t q = 0; q < p.nprob; ++q) p.rec_sub[slot * p.nprob + q] = sub[q];
+        }
+      }
+      // wave-uniform bookkeeping of the L smallest distinct objectives seen by this wave
+      unsigned long long m = __ballot(pass);
+      bool changed = false;
+      while (m) {
+        const int l = __ffsll(m) - 1;
+        m &= m - 1;
+        const double o = __shfl(tot, l);
+        bool present = false;
+        for (uint32_t i = 0; i < nlvl; ++i) present |= (lvl[i] == o);
+        if (present || !(nlvl < p.L || o < lvl[nlvl - 1])) continue;
+        uint32_t pos = nlvl < p.L ? nlvl++ : nlvl - 1;
+        while (pos > 0 && lvl[pos - 1] > o) {
+          lvl[pos] = lvl[pos - 1];
+          --pos;
+        }
+        lvl[pos] = o;
+        changed = true;
+      }
+      if (changed && nlvl == p.L && lane == 0) {
+        const unsigned long long k = okey(lvl[p.L - 1]);
+        if (k < g) atomicMin(p.thr_key, k);
+      }
+    }
+  }
+  if (lane == 0 && scored) atomicAdd(p.scored, scored);
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void score_lane_kernel(ScoreParams p) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = threadIdx.x >> 6;
+  uint8_t *wl = lds + wave * p.wave_lds;
+  uint8_t *col = wl + p.off_row + lane;
+  uint32_t *hl = reinterpret_cast<uint32_t *>(wl + p.off_hist) + lane;
+  const DevProblem &pr = p.prob;
+  const uint32_t nb = pr.S * pr.K * pr.K;
+  const uint64_t b = (static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + wave) * 64 + lane;
+  const bool active = b < p.B;
+  for (uint32_t i = 0; i < pr.N; ++i)
+    col[i << 6] = active ? static_cast<uint8_t>(p.species[b * pr.N + i] * pr.F) : 0;
+  const double obj = score_lane<KM>(col, pr, hl, active && p.bonds ? p.bonds + b * nb : nullptr,
+                                    active && p.sro ? p.sro + b * nb : nullptr);
+  if (active) p.objective[b] = obj;
+}
+
+template <int KM> hipError_t launch_run_lane_k(const RunParams &p, uint32_t blocks, uint32_t W,
+                                               hipStream_t s) {
+  const size_t lds = static_cast<size_t>(W) * p.wave_lds;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&run_lane_kernel<KM>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(run_lane_kernel<KM>, dim3(blocks), dim3(64 * W), lds, s, p);
+  return hipGetLastError();
+}
+
+template <int KM> hipError_t launch_score_lane_k(const ScoreParams &p, hipStream_t s) {
+  const uint32_t W = 1;
+  const size_t lds = p.wave_lds;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&score_lane_kernel<KM>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  const uint32_t blocks = (p.B + 63) / 64;
+  hipLaunchKernelGGL(score_lane_kernel<KM>, dim3(blocks), dim3(64 * W), lds, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+uint32_t lane_field_bits(uint32_t kmax) { return kmax == 0 ? 8u : (32 / kmax > 8 ? 8u : 32 / kmax); }
+
+hipError_t launch_run_lane(const RunParams &p, uint32_t kmax, uint32_t blocks,
+                           uint32_t waves_per_block, hipStream_t s) {
+  switch (kmax) {
+    case 1: case 2: return launch_run_lane_k<2>(p, blocks, waves_per_block, s);
+    case 3: return launch_run_lane_k<3>(p, blocks, waves_per_block, s);
+    case 4: return launch_run_lane_k<4>(p, blocks, waves_per_block, s);
+    case 5: return launch_run_lane_k<5>(p, blocks, waves_per_block, s);
+    case 6: return launch_run_lane_k<6>(p, blocks, waves_per_block, s);
+    case 7: return launch_run_lane_k<7>(p, blocks, waves_per_block, s);
+    case 8: return launch_run_lane_k<8>(p, blocks, waves_per_block, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_score_lane(const ScoreParams &p, uint32_t kmax, hipStream_t s) {
+  if (p.B == 0) return hipSuccess;
+  switch (kmax) {
+    case 1: case 2: return launch_score_lane_k<2>(p, s);
+    case 3: return launch_score_lane_k<3>(p, s);
+    case 4: return launch_score_lane_k<4>(p, s);
+    case 5: return launch_score_lane_k<5>(p, s);
+    case 6: return launch_score_lane_k<6>(p, s);
+    case 7: return launch_score_lane_k<7>(p, s);
+    case 8: return launch_score_lane_k<8>(p, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace sqsg

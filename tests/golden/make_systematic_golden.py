@@ -1,10 +1,12 @@
 """Writes systematic_fcc32.json: exhaustive search of the 32-site FCC A16B16 cell, 2 shells
 (weights 1, 1/2), default pair weights, all C(32,16) permutations, solved by the oracle
-(oracle/sqsgen_oracle.c driver, multithreaded).  Records the optimal objective, the sorted
-optimal reported ranks and the next objective level.
+(oracle/sqsgen_oracle.c driver, multithreaded).  Records the optimal objective, the number
+of optimal configurations with a SHA-256 digest of their sorted reported ranks (little-endian
+uint64), and the next objective level.
 
 Run from the repository root: python tests/golden/make_systematic_golden.py
 """
+import hashlib
 import json
 import os
 import sys
@@ -48,13 +50,20 @@ def main():
     best = levels[0]
     ranks = sorted(O.rank_permutation(r[1]) for r in res if r[0] == best)
     second = [r for r in res if r[0] == levels[1]] if len(levels) > 1 else []
-    out = dict(description=__doc__.strip().splitlines()[0], total=total, radii=list(radii),
-               best_objective=best, best_objective_hex=float(best).hex(), num_best=len(ranks),
-               best_ranks=ranks, second_objective=levels[1] if len(levels) > 1 else None,
-               num_second=len(second), seconds=dt, finished=st.finished)
+    r = np.array(ranks, dtype=np.uint64)
+    out = dict(description="exhaustive search of the 32-site FCC A16B16 cell (a = 4.05), 2 shells "
+                           "(weights 1, 1/2), default pair weights, solved by the oracle",
+               total=total, radii=list(radii), best_objective=best,
+               best_objective_hex=float(best).hex(), num_best=len(ranks),
+               best_ranks_sha256=hashlib.sha256(r.astype("<u8").tobytes()).hexdigest(),
+               best_ranks_head=r[:64].tolist(), best_ranks_tail=r[-64:].tolist(),
+               second_objective=levels[1] if len(levels) > 1 else None,
+               second_objective_hex=float(levels[1]).hex() if len(levels) > 1 else None,
+               num_second=len(second), oracle_seconds=dt)
     path = Path(__file__).with_name("systematic_fcc32.json")
     path.write_text(json.dumps(out, indent=1))
-    print(f"{total} permutations in {dt:.1f}s; best {best} x{len(ranks)}; second {out['second_objective']} x{len(second)}")
+    print(f"{total} permutations in {dt:.1f}s; best {best} x{len(ranks)}; "
+          f"second {out['second_objective']} x{len(second)}")
 
 
 if __name__ == "__main__":

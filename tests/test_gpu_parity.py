@@ -68,9 +68,13 @@ WORKLOADS = {
 }
 
 
+KERNELS = {"wave": 1, "lane": 2}
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
 @pytest.mark.parametrize("name", list(WORKLOADS))
-def test_score_batch_bit_exact(gpu, name):
-    print(f"[score_batch] {name}", flush=True)
+def test_score_batch_bit_exact(gpu, name, kernel):
+    print(f"[score_batch] {name} {kernel}", flush=True)
     cfg = parse_config(WORKLOADS[name]())
     job = Job(cfg)
     rng = np.random.default_rng(20261203)
@@ -84,7 +88,7 @@ def test_score_batch_bit_exact(gpu, name):
         B = 64 if info["num_sites"] < 500 else 16
         cands = random_candidates(ref["packed"], ref["bounds"], B, rng)
         cands[0] = ref["packed"]  # the sorted configuration itself
-        bonds, obj, sro = job.score(cands, p)
+        bonds, obj, sro = job.score(cands, p, kernel=KERNELS[kernel])
         for b in range(B):
             rb = O.count_bonds(ref["pairs"], cands[b], ref["S"], ref["K"])
             assert np.array_equal(bonds[b].astype(np.uint64), rb), f"bonds of candidate {b}"
@@ -191,3 +195,67 @@ def test_mc_tie_set_complete_16(gpu):
     assert sorted(got) == sorted(ref_levels)
     for o in got:
         assert got[o] == ref_levels[o], f"level {o}"
+
+
+def _pack_signature(pack):
+    return [(o, sorted((tuple(r.species), r.candidate) for r in rs)) for o, rs in pack]
+
+
+@pytest.mark.parametrize("name", ["fcc32_3shells", "fcc256_quinary", "rocksalt216_split",
+                                  "hcp128_binary"])
+def test_variants_agree_mc(gpu, name):
+    """candidate t is the same configuration in every kernel (counter-based streams), so the
+    wave and lane kernels must return identical result packs for the same seed"""
+    print(f"[variants] {name}", flush=True)
+    d = WORKLOADS[name]()
+    d["iterations"] = 200_000
+    d["chunk_size"] = 50_000
+    packs = []
+    for kernel in (1, 2):
+        job = Job(parse_config(d))
+        job.set_kernel(kernel)
+        job.run()
+        packs.append(_pack_signature(job.results()))
+        assert job.timing()[2] == 200_000
+        job.close()
+    assert packs[0] == packs[1]
+
+
+def test_variants_agree_systematic(gpu):
+    d = W.reference_test_16()
+    del d["pair_weights"]
+    packs = []
+    for kernel in (1, 2):
+        job = Job(parse_config(d))
+        job.set_kernel(kernel)
+        job.set_launch_size(1000)
+        job.run()
+        packs.append(_pack_signature(job.results()))
+        job.close()
+    assert packs[0] == packs[1]
+
+
+def test_systematic_32_optimal_rank_set(gpu):
+    """all C(32,16) = 601,080,390 permutations of the 32-site A16B16 cell (2 shells): same
+    optimal objective and the same set of optimal reported ranks as the oracle's exhaustive
+    search (tests/golden/systematic_fcc32.json, 17,271,296 optimal ranks)"""
+    import hashlib
+    import json
+    from pathlib import Path
+
+    golden = json.loads((Path(__file__).parent / "golden" / "systematic_fcc32.json").read_text())
+    cfg = parse_config(W.systematic_32())
+    assert cfg.iterations == golden["total"]
+    assert np.array_equal(cfg.shell_radii[0], golden["radii"])
+    job = Job(cfg)
+    job.run()
+    assert job.timing()[2] == golden["total"]
+    pack = job.results()
+    assert float(pack.level_objectives[0]).hex() == golden["best_objective_hex"]
+    assert float(pack.level_objectives[1]).hex() == golden["second_objective_hex"]
+    n_best = int(pack.level_start[1])
+    assert n_best == golden["num_best"]
+    assert int(pack.level_start[2] - pack.level_start[1]) == golden["num_second"]
+    ranks = np.sort(pack.arrays(0, n_best, species=False, ranks=True)["rank"])
+    assert ranks[:64].tolist() == golden["best_ranks_head"]
+    assert hashlib.sha256(ranks.astype("<u8").tobytes()).hexdigest() == golden["best_ranks_sha256"]
