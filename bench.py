@@ -69,7 +69,19 @@ def main() -> int:
                     help="launches of the plain wave-per-candidate kernel timed for comparison")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n, "launches": 3}))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: candidates are independent, so N GPUs would "
               "each run disjoint RNG streams / rank ranges and merge their tie sets on the "

@@ -180,27 +180,32 @@ void Problem::upload_device() {
 
 void Problem::build_lane(uint32_t field_bits) {
   F = field_bits;
-  const uint32_t cap = (1u << F) - 1u;
-  // upper neighbours per (shell, site), ascending j, split into groups of at most `cap`
+  const uint32_t cap = (1u << F) - 1u;  // real neighbours per group (F-bit fields)
+  const uint32_t dummy = N << 6;          // byte offset of the dummy site's row
+  // upper neighbours per (shell, site), ascending j, in groups of at most `cap`; a group is
+  // a header block {i * 64, data blocks} followed by 4-entry blocks of j * 64 padded with
+  // the dummy site
   std::vector<std::vector<uint32_t>> nb(size_t(S) * N);
   for (size_t q = 0; q < pi.size(); ++q) nb[size_t(ps[q]) * N + pi[q]].push_back(pj[q]);
   lane_stream.clear();
   lane_off.assign(S + 1, 0);
   for (uint32_t s = 0; s < S; ++s) {
-    lane_off[s] = static_cast<uint32_t>(lane_stream.size());
+    lane_off[s] = static_cast<uint32_t>(lane_stream.size() / 4);
     for (uint32_t i = 0; i < N; ++i) {
       const auto &js = nb[size_t(s) * N + i];
       for (size_t k = 0; k < js.size(); k += cap) {
         const uint32_t cnt = static_cast<uint32_t>(std::min<size_t>(cap, js.size() - k));
-        lane_stream.push_back(i | (cnt << 16));
-        lane_stream.insert(lane_stream.end(), js.begin() + k, js.begin() + k + cnt);
+        const uint32_t blocks = (cnt + 3) / 4;
+        lane_stream.insert(lane_stream.end(), {i << 6, blocks, 0u, 0u});
+        for (uint32_t e = 0; e < 4 * blocks; ++e)
+          lane_stream.push_back(e < cnt ? (js[k + e] << 6) : dummy);
       }
     }
   }
-  lane_off[S] = static_cast<uint32_t>(lane_stream.size());
+  lane_off[S] = static_cast<uint32_t>(lane_stream.size() / 4);
   std::vector<uint8_t> sF(species_packed.size());
   for (size_t i = 0; i < sF.size(); ++i) sF[i] = static_cast<uint8_t>(species_packed[i] * F);
-  upload(d_lane_stream, lane_stream);
+  upload(d_lane_stream, lane_stream, 16);
   upload(d_lane_off, lane_off);
   upload(d_species0F, sF, 16);
 }
@@ -432,10 +437,10 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     p.gen_group = 64;
     for (uint32_t q = 0; q < p.nprob; ++q) {
       p.off_rows[q] = off;
-      off = align(off + 64 * job->problems[q].N);
+      off = align(off + 64 * (job->problems[q].N + 1));
     }
     p.off_hist = off;
-    off = align(off + job->kmax * lane_field_bits(job->kmax) * 64 * 4);
+    off = align(off + job->kmax * ((job->kmax + 1) / 2) * 64 * 4);
   } else {
     uint32_t G = 64;
     while (G > 8 && static_cast<size_t>(G) * stride_sum > 24576) G >>= 1;
@@ -502,7 +507,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   job->launches = 0;
   job->scored = 0;
   const int variant = resolve_kernel(job, job->kernel);
-  const uint32_t W = variant == SQSG_KERNEL_SPARSE_LANE ? 2 : 4;
+  const uint32_t W = variant == SQSG_KERNEL_SPARSE_LANE ? 1 : 4;
   uint32_t blocks = 0;
   RunParams p = make_run_params(job, variant, W, &blocks);
   const uint64_t launch = job->launch_size ? job->launch_size : default_launch_size(job);
@@ -617,8 +622,8 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   if (variant == SQSG_KERNEL_SPARSE_LANE) {
     sp.off_row = 0;
-    sp.off_hist = align(64 * P.N);
-    sp.wave_lds = align(sp.off_hist + job->kmax * lane_field_bits(job->kmax) * 256);
+    sp.off_hist = align(64 * (P.N + 1));
+    sp.wave_lds = align(sp.off_hist + job->kmax * ((job->kmax + 1) / 2) * 256);
     if (sp.wave_lds > 160 * 1024) throw std::out_of_range("problem exceeds LDS");
     SQSG_HIP(launch_score_lane(sp, job->kmax, s ? s : job->stream));
     return;

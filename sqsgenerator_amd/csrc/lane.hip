@@ -19,11 +19,23 @@
 namespace sqsg {
 namespace {
 
+// F: bits per neighbour-count field.  K * F <= 30 keeps bits 30..31 free for the padding
+// entries of a group (at most 3, they point to a dummy site whose value is K * F).  The
+// histogram keeps W = ceil(K / 2) words of two 16-bit fields per species row; the row
+// address is derived from the site value a * F, so W * 256 must be a multiple of F.
+constexpr uint32_t lane_bits(int km) {
+  const uint32_t w = (km + 1) / 2;
+  uint32_t f = 30u / static_cast<uint32_t>(km);
+  if (f > 8) f = 8;
+  while (f > 1 && (w * 256u) % f) --f;
+  return f;
+}
+
 template <int KM> struct LaneShape {
-  static constexpr uint32_t F = (32 / KM) > 8 ? 8u : static_cast<uint32_t>(32 / KM);
-  static constexpr uint32_t W = (KM + 1) / 2;  // 16-bit field pairs per histogram row
+  static constexpr uint32_t F = lane_bits(KM);
+  static constexpr uint32_t W = (KM + 1) / 2;
   static constexpr uint32_t MASK = (1u << F) - 1u;
-  static_assert(W <= F, "histogram row must fit its stride");
+  static constexpr uint32_t ROW_BYTES_PER_UNIT = W * 256u / F;  // bytes per unit of a * F
 };
 
 // column access: element of site i of this lane's candidate
@@ -87,8 +99,8 @@ __device__ void next_perm_col(uint8_t *col, uint32_t lo, uint32_t w) {
 
 template <int KM>
 __device__ __forceinline__ uint32_t field_count(const uint32_t *hl, uint32_t a, uint32_t b) {
-  constexpr uint32_t F = LaneShape<KM>::F;
-  return (hl[(a * F + (b >> 1)) << 6] >> ((b & 1u) << 4)) & 0xffffu;
+  constexpr uint32_t W = LaneShape<KM>::W;
+  return (hl[(a * W + (b >> 1)) << 6] >> ((b & 1u) << 4)) & 0xffffu;
 }
 
 // Bond histogram + objective of the lane's candidate (`col`), shell by shell.  `hl` is
@@ -99,35 +111,44 @@ __device__ double score_lane(const uint8_t *col, const DevProblem &pr, uint32_t 
   using Sh = LaneShape<KM>;
   constexpr uint32_t F = Sh::F, W = Sh::W, MASK = Sh::MASK;
   const uint32_t K = pr.K;
-  const uint32_t *st = pr.lane_stream;
+  // the neighbour stream is read with in-order vector loads (bounds-checked: reads past
+  // the end return zeros), so loads of the next block overlap the LDS gathers of this one
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t *>(pr.lane_stream), static_cast<short>(0),
+      static_cast<int>(pr.lane_off[pr.S] * 16u), 0x00020000);
+  uint8_t *hb = reinterpret_cast<uint8_t *>(hl);
   double obj = 0.0;
   for (uint32_t s = 0; s < pr.S; ++s) {
 #pragma unroll
     for (uint32_t a = 0; a < KM; ++a)
 #pragma unroll
       for (uint32_t w = 0; w < W; ++w)
-        if (a < K) hl[(a * F + w) << 6] = 0u;
-    const uint32_t e1 = pr.lane_off[s + 1];
-    for (uint32_t e = pr.lane_off[s]; e < e1;) {
-      const uint32_t hdr = st[e];
-      const uint32_t i = hdr & 0xffffu, cnt = hdr >> 16;
-      const uint32_t *js = st + e + 1;
-      e += 1 + cnt;
-      const uint32_t a6 = col[i << 6];
+        if (a < K) hl[(a * W + w) << 6] = 0u;
+    uint32_t blk = pr.lane_off[s];
+    const uint32_t end = pr.lane_off[s + 1];
+    auto hdr = __builtin_amdgcn_raw_buffer_load_b128(rs, blk * 16u, 0, 0);
+    while (blk < end) {
+      const uint32_t i_off = hdr[0], nb = hdr[1];
+      ++blk;
+      const uint32_t a6 = col[i_off];
       uint32_t acc = 0;
-      uint32_t k = 0;
-      for (; k + 4 <= cnt; k += 4) {
-        const uint32_t b0 = col[js[k] << 6], b1 = col[js[k + 1] << 6];
-        const uint32_t b2 = col[js[k + 2] << 6], b3 = col[js[k + 3] << 6];
+      auto d0 = __builtin_amdgcn_raw_buffer_load_b128(rs, blk * 16u, 0, 0);
+      auto d1 = __builtin_amdgcn_raw_buffer_load_b128(rs, (blk + 1) * 16u, 0, 0);
+      for (uint32_t t = 0; t < nb; ++t) {
+        const auto d2 = __builtin_amdgcn_raw_buffer_load_b128(rs, (blk + 2) * 16u, 0, 0);
+        const uint32_t b0 = col[d0[0]], b1 = col[d0[1]], b2 = col[d0[2]], b3 = col[d0[3]];
         acc += ((1u << b0) + (1u << b1)) + ((1u << b2) + (1u << b3));
+        d0 = d1;
+        d1 = d2;
+        ++blk;
       }
-      for (; k < cnt; ++k) acc += 1u << col[js[k] << 6];
-      uint32_t *h = hl + (a6 << 6);
+      hdr = d0;  // the block after a group's data is the next group's header
+      uint8_t *h = hb + a6 * Sh::ROW_BYTES_PER_UNIT;
 #pragma unroll
       for (uint32_t w = 0; w < W; ++w) {
         uint32_t v = (acc >> (2 * w * F)) & MASK;
         if (2 * w + 1 < KM) v |= ((acc >> ((2 * w + 1) * F)) & MASK) << 16;
-        atomicAdd(h + (w << 6), v);
+        atomicAdd(reinterpret_cast<uint32_t *>(h + (w << 8)), v);
       }
     }
     // this shell's objective terms, optimization.h:85-100
@@ -170,6 +191,9 @@ __global__ __launch_bounds__(256) void run_lane_kernel(RunParams p) {
   double lvl[kMaxLevels];
   uint32_t nlvl = 0;
   unsigned long long scored = 0;
+  // dummy site N of every problem: padding entries of the neighbour stream read it
+  for (uint32_t q = 0; q < p.nprob; ++q)
+    wl[p.off_rows[q] + (p.prob[q].N << 6) + lane] = static_cast<uint8_t>(p.prob[q].K * p.prob[q].F);
   for (uint64_t seg = gw; seg < nseg; seg += GW) {
     const uint64_t sbeg = p.cand_begin + seg * per_seg;
     for (uint32_t r = 0; r < p.rounds; ++r) {
@@ -265,6 +289,7 @@ __global__ __launch_bounds__(256) void score_lane_kernel(ScoreParams p) {
   const bool active = b < p.B;
   for (uint32_t i = 0; i < pr.N; ++i)
     col[i << 6] = active ? static_cast<uint8_t>(p.species[b * pr.N + i] * pr.F) : 0;
+  col[pr.N << 6] = static_cast<uint8_t>(pr.K * pr.F);
   const double obj = score_lane<KM>(col, pr, hl, active && p.bonds ? p.bonds + b * nb : nullptr,
                                     active && p.sro ? p.sro + b * nb : nullptr);
   if (active) p.objective[b] = obj;
@@ -299,7 +324,7 @@ template <int KM> hipError_t launch_score_lane_k(const ScoreParams &p, hipStream
 
 }  // namespace
 
-uint32_t lane_field_bits(uint32_t kmax) { return kmax == 0 ? 8u : (32 / kmax > 8 ? 8u : 32 / kmax); }
+uint32_t lane_field_bits(uint32_t kmax) { return lane_bits(kmax < 2 ? 2 : static_cast<int>(kmax)); }
 
 hipError_t launch_run_lane(const RunParams &p, uint32_t kmax, uint32_t blocks,
                            uint32_t waves_per_block, hipStream_t s) {
