@@ -714,6 +714,68 @@ class Job:
     def results(self) -> "SqsResultPack":
         return SqsResultPack._from_job(self)
 
+    def autotune(self) -> tuple[int, float]:
+        """times every kernel variant on a short run and keeps the fastest (clears results)"""
+        k, r = C.c_int(), C.c_double()
+        L.check(L.lib().sqsg_autotune(self.handle, C.byref(k), C.byref(r)))
+        return k.value, r.value
+
+
+def batch_run(jobs: list["Job"], use_graph: bool = True) -> dict:
+    """Runs every job to completion in shared persistent launches (the batched form)."""
+    arr = (C.c_void_p * len(jobs))(*[j.handle for j in jobs])
+    rep = L.BatchReport()
+    L.check(L.lib().sqsg_batch_run(arr, len(jobs), int(use_graph), C.byref(rep)))
+    return dict(kernel_ms=rep.kernel_ms, candidates=rep.candidates, launches=rep.launches,
+                work_items=rep.work_items)
+
+
+class Server:
+    """Continuous batching: submitted jobs are admitted by device-memory footprint between
+    persistent steps and advance together with the jobs already resident."""
+
+    def __init__(self, hbm_budget_bytes: int = 8 << 30, max_active: int = 64,
+                 slice_candidates: int = 0):
+        opt = L.ServerOptions(hbm_budget_bytes, max_active, slice_candidates)
+        h = C.c_void_p()
+        L.check(L.lib().sqsg_server_create(C.byref(opt), C.byref(h)))
+        self.handle = h
+        self._jobs = {}
+
+    def submit(self, job: "Job") -> int:
+        t = C.c_uint64()
+        L.check(L.lib().sqsg_server_submit(self.handle, job.handle, C.byref(t)))
+        self._jobs[t.value] = job
+        return t.value
+
+    def step(self) -> tuple[int, int]:
+        a, q = C.c_int(), C.c_int()
+        L.check(L.lib().sqsg_server_step(self.handle, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def poll(self, ticket: int) -> dict:
+        d, ts, ta, td = C.c_int(), C.c_double(), C.c_double(), C.c_double()
+        L.check(L.lib().sqsg_server_poll(self.handle, ticket, C.byref(d), C.byref(ts),
+                                         C.byref(ta), C.byref(td)))
+        return dict(done=bool(d.value), t_submit=ts.value, t_admit=ta.value, t_done=td.value)
+
+    def drain(self) -> None:
+        while True:
+            a, q = self.step()
+            if a == 0 and q == 0:
+                return
+
+    def close(self) -> None:
+        if self.handle:
+            L.lib().sqsg_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def _stats_dict(s) -> dict:
     return dict(finished=s.finished, working=s.working, best_rank=s.best_rank,

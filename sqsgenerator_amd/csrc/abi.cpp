@@ -362,4 +362,53 @@ int sqsg_result_sro_bulk(sqsg_job *job, uint64_t first, uint64_t count, double *
   });
 }
 
+int sqsg_batch_run(sqsg_job **jobs, uint32_t num_jobs, int use_graph, sqsg_batch_report *report) {
+  return guard([&] {
+    need(jobs, "jobs");
+    for (uint32_t k = 0; k < num_jobs; ++k) need(jobs[k], "job");
+    sqsg::batch_run(jobs, num_jobs, use_graph, report);
+  });
+}
+
+int sqsg_autotune(sqsg_job *job, int *chosen, double *cand_per_s) {
+  return guard([&] {
+    need(job, "job");
+    sqsg::autotune(job, chosen, cand_per_s);
+  });
+}
+
+int sqsg_server_create(const sqsg_server_options *opt, sqsg_server **out) {
+  return guard([&] {
+    need(out, "out");
+    sqsg_server_options o{};
+    if (opt) o = *opt;
+    *out = sqsg::server_create(o);
+  });
+}
+
+int sqsg_server_submit(sqsg_server *srv, sqsg_job *job, uint64_t *ticket) {
+  return guard([&] {
+    need(srv, "server"), need(job, "job");
+    const uint64_t t = sqsg::server_submit(srv, job);
+    if (ticket) *ticket = t;
+  });
+}
+
+int sqsg_server_step(sqsg_server *srv, int *active_jobs, int *queued_jobs) {
+  return guard([&] {
+    need(srv, "server");
+    sqsg::server_step(srv, active_jobs, queued_jobs);
+  });
+}
+
+int sqsg_server_poll(sqsg_server *srv, uint64_t ticket, int *done, double *t_submit,
+                     double *t_admit, double *t_done) {
+  return guard([&] {
+    need(srv, "server");
+    sqsg::server_poll(srv, ticket, done, t_submit, t_admit, t_done);
+  });
+}
+
+void sqsg_server_destroy(sqsg_server *srv) { sqsg::server_destroy(srv); }
+
 }  // extern "C"

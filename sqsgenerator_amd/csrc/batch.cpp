@@ -1,0 +1,418 @@
+// Batched persistent execution and continuous-batching service.
+//
+// BatchEngine keeps a set of resident jobs (their inputs, thresholds and record sinks stay
+// in device memory) and advances all of them with one persistent launch per step: each
+// step gives every active job its next slice of candidates, split into work items that the
+// persistent wavefronts pull from a shared counter.  Jobs are grouped by their largest
+// species count (the lane kernel is specialised on it); each group has fixed device tables
+// and a captured graph {counter reset kernel, persistent kernel} that is replayed every
+// step.  Between steps the host drains the record sinks, tightens each job's threshold and
+// admits or retires jobs, which is what the serving loop builds on.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+#include <memory>
+
+#include "job.hpp"
+#include "kernels.hpp"
+
+namespace sqsg {
+
+namespace {
+
+constexpr uint32_t kGroupJobs = 256;      // resident jobs per species-count group
+constexpr uint32_t kGroupItems = 1u << 17;
+
+struct Slot {
+  sqsg_job *job = nullptr;
+  RunParams p{};
+  RecordSink sink;
+  DevBuf thr, count, scored;
+  uint64_t next = 0;         // first candidate of the next slice
+  uint64_t slice = 0;        // candidates per step
+  uint64_t item = 0;         // candidates per work item
+  uint64_t issued_end = 0;   // end of the slice issued in the current step
+  size_t footprint = 0;      // device bytes held
+  bool done = false;
+};
+
+struct Group {
+  uint32_t kmax = 0;
+  DevBuf d_jobs, d_items, d_counts, d_next;
+  hipGraphExec_t exec = nullptr;
+  uint32_t graph_wave_lds = 0, graph_blocks = 0;
+  std::vector<Slot *> members;
+  ~Group() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+size_t job_footprint(const sqsg_job *job) {
+  size_t b = 0;
+  for (const auto &P : job->problems)
+    b += P.pi.size() * 4 + P.lane_stream.size() * 4 + P.pref.size() * 24 + 2 * P.N + 1024;
+  return b;
+}
+
+}  // namespace
+
+class BatchEngine {
+ public:
+  explicit BatchEngine(bool use_graph) : use_graph_(use_graph) {
+    SQSG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SQSG_HIP(hipEventCreate(&ev0_));
+    SQSG_HIP(hipEventCreate(&ev1_));
+  }
+  ~BatchEngine() {
+    groups_.clear();
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  // Makes a job resident; its run state is reset exactly as sqsg_run would.
+  size_t add(sqsg_job *job) {
+    begin_run(job);
+    if (resolve_kernel(job, job->kernel == SQSG_KERNEL_SPARSE_WAVE ? SQSG_KERNEL_AUTO
+                                                                   : job->kernel) !=
+        SQSG_KERNEL_SPARSE_LANE)
+      throw std::invalid_argument("batched execution needs a job the lane kernel handles");
+    auto s = std::make_unique<Slot>();
+    s->job = job;
+    uint32_t blocks = 0;
+    s->p = make_run_params(job, SQSG_KERNEL_SPARSE_LANE, 1, &blocks);
+    s->slice = job->launch_size ? job->launch_size : default_launch_size(job);
+    uint64_t work = 0;
+    for (const auto &P : job->problems) work += P.pi.size() + 64;
+    if (s->p.systematic) {
+      s->p.rounds = static_cast<uint32_t>(std::clamp<uint64_t>(
+          (uint64_t(1) << 21) / (64 * std::max<uint64_t>(work, 1)), 1, 256));
+    } else {
+      s->p.rounds = 1;
+    }
+    const uint64_t seg = 64ull * s->p.rounds;
+    s->item = std::max<uint64_t>(seg, ((uint64_t(1) << 22) / std::max<uint64_t>(work, 1)) / seg * seg);
+    s->slice = std::max(s->slice, s->item);
+    s->p.rec_stride = job->stride;
+    s->sink.ensure(static_cast<uint32_t>(std::min<uint64_t>(s->slice, 1u << 14)), job->stride,
+                   s->p.nprob);
+    s->thr.alloc(sizeof(unsigned long long));
+    s->count.alloc(sizeof(unsigned int));
+    s->scored.alloc(sizeof(unsigned long long));
+    const unsigned long long inf = objective_key(std::numeric_limits<double>::infinity());
+    SQSG_HIP(hipMemcpy(s->thr.as<void>(), &inf, sizeof(inf), hipMemcpyHostToDevice));
+    s->p.thr_key = s->thr.as<unsigned long long>();
+    s->p.rec_count = s->count.as<unsigned int>();
+    s->p.scored = s->scored.as<unsigned long long>();
+    s->footprint = job_footprint(job) + s->sink.cap * (24 + job->stride);
+    Group &g = group(job->kmax);
+    if (g.members.size() >= kGroupJobs) throw std::out_of_range("too many resident jobs");
+    g.members.push_back(s.get());
+    const size_t fp = s->footprint;
+    slots_.push_back(std::move(s));
+    return fp;
+  }
+
+  bool idle() const { return slots_.empty(); }
+  size_t resident() const { return slots_.size(); }
+
+  // One persistent launch over every active job; finished jobs are finalised and returned.
+  std::vector<sqsg_job *> step() {
+    std::vector<sqsg_job *> finished;
+    if (slots_.empty()) return finished;
+    const auto t0 = std::chrono::steady_clock::now();
+    SQSG_HIP(hipEventRecord(ev0_, stream_));
+    for (auto &[k, g] : groups_) {
+      if (g->members.empty()) continue;
+      launch_group(*g);
+    }
+    SQSG_HIP(hipEventRecord(ev1_, stream_));
+    SQSG_HIP(hipStreamSynchronize(stream_));
+    float ms = 0;
+    SQSG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    kernel_ms += ms;
+    ++launches;
+    for (auto &s : slots_) {
+      sqsg_job *job = s->job;
+      if (s->issued_end == s->next) {
+        if (s->next >= job->iterations || job->cancel.load()) s->done = true;
+        continue;
+      }
+      unsigned int count = 0;
+      unsigned long long scored = 0;
+      SQSG_HIP(hipMemcpyAsync(&count, s->count.as<void>(), sizeof(count), hipMemcpyDeviceToHost,
+                              stream_));
+      SQSG_HIP(hipMemcpyAsync(&scored, s->scored.as<void>(), sizeof(scored),
+                              hipMemcpyDeviceToHost, stream_));
+      SQSG_HIP(hipStreamSynchronize(stream_));
+      job->launches++;
+      job->kernel_ms += ms;  // shared launch: every resident job saw the whole step
+      if (count > s->sink.cap) {  // widen and issue the same slice again next step
+        s->sink.ensure(count + count / 4, job->stride, s->p.nprob);
+        s->footprint = job_footprint(job) + s->sink.cap * (24 + job->stride);
+        continue;
+      }
+      if (scored != s->issued_end - s->next)
+        throw DeviceError("persistent launch scored an unexpected candidate count");
+      job->scored += scored;
+      candidates += scored;
+      s->sink.drain(job, count, s->p.nprob, stream_);
+      const unsigned long long hk = objective_key(job->vkeep);
+      unsigned long long dk = 0;
+      SQSG_HIP(hipMemcpy(&dk, s->thr.as<void>(), sizeof(dk), hipMemcpyDeviceToHost));
+      if (hk < dk) SQSG_HIP(hipMemcpy(s->thr.as<void>(), &hk, sizeof(hk), hipMemcpyHostToDevice));
+      s->next = s->issued_end;
+      job->stats.finished = job->scored;
+      if (s->next >= job->iterations || job->cancel.load()) s->done = true;
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    for (auto it = slots_.begin(); it != slots_.end();) {
+      if (!(*it)->done) {
+        ++it;
+        continue;
+      }
+      sqsg_job *job = (*it)->job;
+      finalize_results(job);
+      job->stats.timing_loop_ns +=
+          std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+      job->stats.timing_total_ns = job->stats.timing_loop_ns;
+      Group &g = group(job->kmax);
+      g.members.erase(std::find(g.members.begin(), g.members.end(), it->get()));
+      finished.push_back(job);
+      it = slots_.erase(it);
+    }
+    for (auto &s : slots_)
+      s->job->stats.timing_loop_ns +=
+          std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+    return finished;
+  }
+
+  double kernel_ms = 0;
+  uint64_t candidates = 0, launches = 0;
+  uint32_t last_items = 0;
+
+ private:
+  Group &group(uint32_t kmax) {
+    auto &g = groups_[kmax];
+    if (!g) {
+      g = std::make_unique<Group>();
+      g->kmax = kmax;
+      g->d_jobs.alloc(kGroupJobs * sizeof(RunParams));
+      g->d_items.alloc(kGroupItems * sizeof(WorkItem));
+      g->d_counts.alloc(2 * sizeof(uint32_t));
+      g->d_next.alloc(sizeof(unsigned int));
+    }
+    return *g;
+  }
+
+  void launch_group(Group &g) {
+    std::vector<RunParams> table(g.members.size());
+    std::vector<WorkItem> items;
+    uint32_t wave_lds = 0;
+    for (uint32_t k = 0; k < g.members.size(); ++k) {
+      Slot *s = g.members[k];
+      s->sink.bind(s->p);
+      table[k] = s->p;
+      wave_lds = std::max(wave_lds, s->p.wave_lds);
+      const uint64_t end = std::min(s->job->iterations, s->next + s->slice);
+      s->issued_end = s->job->cancel.load() ? s->next : end;
+      for (uint64_t b = s->next; b < s->issued_end; b += s->item) {
+        if (items.size() == kGroupItems) throw std::out_of_range("work item table full");
+        items.push_back(WorkItem{k, 0, b, std::min(s->issued_end, b + s->item)});
+      }
+    }
+    last_items = static_cast<uint32_t>(items.size());
+    const uint32_t n = last_items;
+    SQSG_HIP(hipMemcpyAsync(g.d_jobs.as<void>(), table.data(), table.size() * sizeof(RunParams),
+                            hipMemcpyHostToDevice, stream_));
+    if (n)
+      SQSG_HIP(hipMemcpyAsync(g.d_items.as<void>(), items.data(), n * sizeof(WorkItem),
+                              hipMemcpyHostToDevice, stream_));
+    const uint32_t counts[2] = {n, static_cast<uint32_t>(g.members.size())};
+    SQSG_HIP(hipMemcpyAsync(g.d_counts.as<void>(), counts, sizeof(counts), hipMemcpyHostToDevice,
+                            stream_));
+    SQSG_HIP(hipStreamSynchronize(stream_));  // `table`/`items` are stack buffers
+    const size_t block_lds = wave_lds;
+    uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / std::max<size_t>(block_lds, 1));
+    per_cu = std::max(1u, std::min(per_cu, 16u));
+    const uint32_t blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
+    BatchParams bp{};
+    bp.jobs = g.d_jobs.as<const RunParams>();
+    bp.items = g.d_items.as<const WorkItem>();
+    bp.num_items = g.d_counts.as<const uint32_t>();
+    bp.num_jobs = g.d_counts.as<const uint32_t>() + 1;
+    bp.wave_lds = wave_lds;
+    bp.next_item = g.d_next.as<unsigned int>();
+    SQSG_HIP(prepare_batch_lane(g.kmax, block_lds));
+    if (!use_graph_) {
+      SQSG_HIP(launch_batch_reset(bp, kGroupJobs, stream_));
+      SQSG_HIP(launch_batch_lane(bp, g.kmax, blocks, 1, stream_));
+      return;
+    }
+    if (!g.exec || g.graph_wave_lds != wave_lds || g.graph_blocks != blocks) {
+      // the graph covers every table row (num_jobs is a kernel argument of the reset)
+      if (g.exec) SQSG_HIP(hipGraphExecDestroy(g.exec));
+      g.exec = nullptr;
+      hipGraph_t graph = nullptr;
+      SQSG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      const hipError_t e1 = launch_batch_reset(bp, kGroupJobs, stream_);
+      const hipError_t e2 = launch_batch_lane(bp, g.kmax, blocks, 1, stream_);
+      SQSG_HIP(hipStreamEndCapture(stream_, &graph));
+      SQSG_HIP(e1);
+      SQSG_HIP(e2);
+      const hipError_t e3 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      SQSG_HIP(e3);
+      g.graph_wave_lds = wave_lds;
+      g.graph_blocks = blocks;
+    }
+    SQSG_HIP(hipGraphLaunch(g.exec, stream_));
+  }
+
+  bool use_graph_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  std::vector<std::unique_ptr<Slot>> slots_;
+  std::map<uint32_t, std::unique_ptr<Group>> groups_;
+};
+
+void batch_run(sqsg_job **jobs, uint32_t n, int use_graph, sqsg_batch_report *report) {
+  BatchEngine eng(use_graph != 0);
+  for (uint32_t k = 0; k < n; ++k) eng.add(jobs[k]);
+  uint32_t max_items = 0;
+  while (!eng.idle()) {
+    eng.step();
+    max_items = std::max(max_items, eng.last_items);
+  }
+  if (report) {
+    report->kernel_ms = eng.kernel_ms;
+    report->candidates = eng.candidates;
+    report->launches = eng.launches;
+    report->work_items = max_items;
+  }
+}
+
+// Times every applicable kernel variant on a short launch of the job and keeps the fastest.
+// The probe runs overwrite the job's results.
+void autotune(sqsg_job *job, int *chosen, double *rate) {
+  ensure_device(job);
+  std::vector<int> variants{SQSG_KERNEL_SPARSE_WAVE};
+  if (job->lane_ok) variants.push_back(SQSG_KERNEL_SPARSE_LANE);
+  const uint64_t saved_it = job->iterations, saved_launch = job->launch_size;
+  const int saved_kernel = job->kernel;
+  uint64_t work = 0;
+  for (const auto &P : job->problems) work += P.pi.size() + 64;
+  const uint64_t probe = std::clamp<uint64_t>((uint64_t(1) << 31) / work, 4096, saved_it);
+  int best = SQSG_KERNEL_SPARSE_WAVE;
+  double best_rate = 0;
+  try {
+    job->iterations = probe;
+    job->launch_size = probe;
+    for (int v : variants) {
+      job->kernel = v;
+      run_job(job, nullptr, nullptr);  // warm-up (module load, first-touch)
+      run_job(job, nullptr, nullptr);
+      const double r = job->scored / (job->kernel_ms * 1e-3);
+      if (r > best_rate) best_rate = r, best = v;
+    }
+  } catch (...) {
+    job->iterations = saved_it, job->launch_size = saved_launch, job->kernel = saved_kernel;
+    throw;
+  }
+  job->iterations = saved_it;
+  job->launch_size = saved_launch;
+  job->kernel = best;
+  reset_results(job);
+  if (chosen) *chosen = best;
+  if (rate) *rate = best_rate;
+}
+
+// ---- serving ---------------------------------------------------------------------------
+
+struct Ticket {
+  sqsg_job *job;
+  double t_submit = -1, t_admit = -1, t_done = -1;
+  bool done = false;
+  size_t footprint = 0;
+};
+
+}  // namespace sqsg
+
+struct sqsg_server {
+  sqsg_server_options opt{};
+  std::unique_ptr<sqsg::BatchEngine> eng;
+  std::deque<uint64_t> queue;
+  std::vector<sqsg::Ticket> tickets;
+  std::map<sqsg_job *, uint64_t> active;
+  size_t resident_bytes = 0;
+  std::chrono::steady_clock::time_point t0;
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace sqsg {
+
+sqsg_server *server_create(const sqsg_server_options &o) {
+  auto s = std::make_unique<sqsg_server>();
+  s->opt = o;
+  if (!s->opt.max_active) s->opt.max_active = 64;
+  if (!s->opt.hbm_budget_bytes) s->opt.hbm_budget_bytes = size_t(8) << 30;
+  s->eng = std::make_unique<BatchEngine>(true);
+  s->t0 = std::chrono::steady_clock::now();
+  return s.release();
+}
+
+uint64_t server_submit(sqsg_server *s, sqsg_job *job) {
+  Ticket t;
+  t.job = job;
+  t.t_submit = s->now();
+  s->tickets.push_back(t);
+  const uint64_t id = s->tickets.size() - 1;
+  s->queue.push_back(id);
+  return id;
+}
+
+// Admission by footprint, then one persistent step over the resident jobs.
+void server_step(sqsg_server *s, int *active, int *queued) {
+  while (!s->queue.empty() && s->active.size() < s->opt.max_active) {
+    Ticket &t = s->tickets[s->queue.front()];
+    ensure_device(t.job);
+    const size_t need = job_footprint(t.job) + (size_t(1) << 14) * (24 + t.job->stride);
+    if (!s->active.empty() && s->resident_bytes + need > s->opt.hbm_budget_bytes) break;
+    if (s->opt.slice_candidates) t.job->launch_size = s->opt.slice_candidates;
+    t.footprint = s->eng->add(t.job);
+    s->resident_bytes += t.footprint;
+    t.t_admit = s->now();
+    s->active[t.job] = s->queue.front();
+    s->queue.pop_front();
+  }
+  if (!s->eng->idle()) {
+    for (sqsg_job *j : s->eng->step()) {
+      const uint64_t id = s->active.at(j);
+      Ticket &t = s->tickets[id];
+      t.done = true;
+      t.t_done = s->now();
+      s->resident_bytes -= std::min(s->resident_bytes, t.footprint);
+      s->active.erase(j);
+    }
+  }
+  if (active) *active = static_cast<int>(s->active.size());
+  if (queued) *queued = static_cast<int>(s->queue.size());
+}
+
+void server_poll(sqsg_server *s, uint64_t id, int *done, double *ts, double *ta, double *td) {
+  if (id >= s->tickets.size()) throw std::out_of_range("unknown ticket");
+  const Ticket &t = s->tickets[id];
+  if (done) *done = t.done ? 1 : 0;
+  if (ts) *ts = t.t_submit;
+  if (ta) *ta = t.t_admit;
+  if (td) *td = t.t_done;
+}
+
+void server_destroy(sqsg_server *s) { delete s; }
+
+}  // namespace sqsg

@@ -68,6 +68,22 @@ struct RunParams {
   unsigned long long *scored;
 };
 
+// A slice of one job's candidate range handed to a persistent wavefront.
+struct WorkItem {
+  uint32_t job;
+  uint32_t pad;
+  uint64_t begin, end;
+};
+
+struct BatchParams {
+  const RunParams *jobs;       // device table, one per job
+  const WorkItem *items;       // device table
+  const uint32_t *num_items;   // device scalars {items, jobs}: a captured graph stays valid
+  const uint32_t *num_jobs;    // while the tables change between replays
+  uint32_t wave_lds;           // LDS of one wavefront: the largest job's layout
+  unsigned int *next_item;     // work-item counter
+};
+
 struct ScoreParams {
   DevProblem prob;
   const uint8_t *species;      // B x N

@@ -471,33 +471,54 @@ hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, 
   return launch_run_wave(p, blocks, W, s);
 }
 
-namespace {
+void RecordSink::ensure(uint32_t n, uint32_t stride, uint32_t nprob) {
+  if (n <= cap) return;
+  cap = n;
+  cand.alloc(size_t(n) * sizeof(uint64_t));
+  obj.alloc(size_t(n) * sizeof(double));
+  sub.alloc(size_t(n) * nprob * sizeof(double));
+  species.alloc(size_t(n) * stride);
+}
 
-struct RecordSink {
-  DevBuf cand, obj, sub, species;
-  uint32_t cap = 0;
-  void ensure(uint32_t n, uint32_t stride, uint32_t nprob) {
-    if (n <= cap) return;
-    cap = n;
-    cand.alloc(size_t(n) * sizeof(uint64_t));
-    obj.alloc(size_t(n) * sizeof(double));
-    sub.alloc(size_t(n) * nprob * sizeof(double));
-    species.alloc(size_t(n) * stride);
-  }
-};
+void RecordSink::bind(RunParams &p) const {
+  p.rec_cap = cap;
+  p.rec_cand = cand.as<uint64_t>();
+  p.rec_obj = obj.as<double>();
+  p.rec_sub = sub.as<double>();
+  p.rec_species = species.as<uint8_t>();
+}
+
+void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob, hipStream_t s) {
+  if (!count) return;
+  const uint32_t stride = job->stride;
+  std::vector<uint64_t> c(count);
+  std::vector<double> o(count), sb(size_t(count) * nprob);
+  std::vector<uint8_t> sp(size_t(count) * stride);
+  SQSG_HIP(hipMemcpyAsync(c.data(), cand.as<void>(), count * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SQSG_HIP(hipMemcpyAsync(o.data(), obj.as<void>(), count * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  SQSG_HIP(hipMemcpyAsync(sb.data(), sub.as<void>(), sb.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  SQSG_HIP(hipMemcpyAsync(sp.data(), species.as<void>(), sp.size(), hipMemcpyDeviceToHost, s));
+  SQSG_HIP(hipStreamSynchronize(s));
+  merge_records(job, c.data(), o.data(), sb.data(), sp.data(), count);
+  for (uint32_t k = 0; k < count; ++k)
+    if (o[k] < job->stats.best_objective ||
+        (o[k] == job->stats.best_objective && c[k] < job->stats.best_rank)) {
+      job->stats.best_objective = o[k];
+      job->stats.best_rank = c[k];
+    }
+}
 
 uint64_t default_launch_size(const sqsg_job *job) {
   uint64_t work = 0;
   for (const auto &P : job->problems) work += P.pi.size() + 64;
-  uint64_t launch = (uint64_t(1) << 34) / std::max<uint64_t>(work, 1);
+  const uint64_t launch = (uint64_t(1) << 34) / std::max<uint64_t>(work, 1);
   return std::clamp<uint64_t>(launch, uint64_t(1) << 16, uint64_t(1) << 24);
 }
 
-}  // namespace
-
-void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
-  using clock = std::chrono::steady_clock;
-  const auto t_start = clock::now();
+void begin_run(sqsg_job *job) {
   ensure_device(job);
   job->cancel.store(false);
   reset_results(job);
@@ -506,6 +527,12 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   job->kernel_ms = 0;
   job->launches = 0;
   job->scored = 0;
+}
+
+void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
+  using clock = std::chrono::steady_clock;
+  const auto t_start = clock::now();
+  begin_run(job);
   const int variant = resolve_kernel(job, job->kernel);
   const uint32_t W = variant == SQSG_KERNEL_SPARSE_LANE ? 1 : 4;
   uint32_t blocks = 0;
@@ -537,11 +564,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     const uint64_t end = std::min(job->iterations, begin + launch);
     p.cand_begin = begin;
     p.cand_end = end;
-    p.rec_cap = sink.cap;
-    p.rec_cand = sink.cand.as<uint64_t>();
-    p.rec_obj = sink.obj.as<double>();
-    p.rec_sub = sink.sub.as<double>();
-    p.rec_species = sink.species.as<uint8_t>();
+    sink.bind(p);
     SQSG_HIP(hipMemsetAsync(d_count.as<void>(), 0, sizeof(unsigned int), job->stream));
     SQSG_HIP(hipMemsetAsync(d_scored.as<void>(), 0, sizeof(unsigned long long), job->stream));
     SQSG_HIP(hipEventRecord(job->ev0, job->stream));
@@ -564,25 +587,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     }
     if (scored != end - begin) throw DeviceError("device scored an unexpected candidate count");
     job->scored += scored;
-    if (count) {
-      std::vector<uint64_t> cand(count);
-      std::vector<double> obj(count), sub(size_t(count) * p.nprob);
-      std::vector<uint8_t> sp(size_t(count) * stride);
-      SQSG_HIP(hipMemcpy(cand.data(), sink.cand.as<void>(), count * sizeof(uint64_t),
-                         hipMemcpyDeviceToHost));
-      SQSG_HIP(hipMemcpy(obj.data(), sink.obj.as<void>(), count * sizeof(double),
-                         hipMemcpyDeviceToHost));
-      SQSG_HIP(hipMemcpy(sub.data(), sink.sub.as<void>(), sub.size() * sizeof(double),
-                         hipMemcpyDeviceToHost));
-      SQSG_HIP(hipMemcpy(sp.data(), sink.species.as<void>(), sp.size(), hipMemcpyDeviceToHost));
-      merge_records(job, cand.data(), obj.data(), sub.data(), sp.data(), count);
-      for (uint32_t k = 0; k < count; ++k)
-        if (obj[k] < job->stats.best_objective ||
-            (obj[k] == job->stats.best_objective && cand[k] < job->stats.best_rank)) {
-          job->stats.best_objective = obj[k];
-          job->stats.best_rank = cand[k];
-        }
-    }
+    sink.drain(job, count, p.nprob, job->stream);
     // keep the device threshold at the tighter of its own bound and the exact host one
     const unsigned long long hk = objective_key(job->vkeep);
     unsigned long long dk = 0;

@@ -143,6 +143,18 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
 // helpers shared with the batched and serving drivers
 RunParams make_run_params(sqsg_job *job, int variant, uint32_t waves_per_block,
                           uint32_t *blocks);
+// Device record sink of one job: candidates at or below the threshold with their
+// objectives and packed species.
+struct RecordSink {
+  DevBuf cand, obj, sub, species;
+  uint32_t cap = 0;
+  void ensure(uint32_t n, uint32_t stride, uint32_t nprob);
+  void bind(RunParams &p) const;
+  // copies `count` records back and merges them into the levels of the job
+  void drain(sqsg_job *job, uint32_t count, uint32_t nprob, hipStream_t s);
+};
+uint64_t default_launch_size(const sqsg_job *job);
+void begin_run(sqsg_job *job);
 // kernel variant a run uses (SQSG_KERNEL_SPARSE_WAVE / _LANE / _DENSE_MFMA)
 int resolve_kernel(const sqsg_job *job, int requested);
 hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, uint32_t blocks,
@@ -160,6 +172,13 @@ double result_objective(const sqsg_job *job, uint64_t r);
 void result_bulk(sqsg_job *job, uint64_t first, uint64_t count, double *objective,
                  uint64_t *candidate, uint8_t *species_z, uint64_t *rank);
 int device_cu_count();
+void batch_run(sqsg_job **jobs, uint32_t n, int use_graph, sqsg_batch_report *report);
+void autotune(sqsg_job *job, int *chosen, double *rate);
+sqsg_server *server_create(const sqsg_server_options &o);
+uint64_t server_submit(sqsg_server *s, sqsg_job *job);
+void server_step(sqsg_server *s, int *active, int *queued);
+void server_poll(sqsg_server *s, uint64_t id, int *done, double *ts, double *ta, double *td);
+void server_destroy(sqsg_server *s);
 void ensure_device(sqsg_job *job);
 
 }  // namespace sqsg

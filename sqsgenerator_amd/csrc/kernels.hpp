@@ -19,6 +19,12 @@ hipError_t launch_run_wave(const RunParams &p, uint32_t blocks, uint32_t waves_p
 hipError_t launch_run_lane(const RunParams &p, uint32_t kmax, uint32_t blocks,
                            uint32_t waves_per_block, hipStream_t stream);
 hipError_t launch_score_lane(const ScoreParams &p, uint32_t kmax, hipStream_t stream);
+// persistent multi-job form over a device table of jobs and work items
+hipError_t launch_batch_lane(const BatchParams &bp, uint32_t kmax, uint32_t blocks,
+                             uint32_t waves_per_block, hipStream_t stream);
+hipError_t launch_batch_reset(const BatchParams &bp, uint32_t max_jobs, hipStream_t stream);
+// raises the dynamic LDS limit of the persistent kernel (called outside graph capture)
+hipError_t prepare_batch_lane(uint32_t kmax, size_t lds);
 // bits per packed neighbour-count field for a largest species count kmax
 uint32_t lane_field_bits(uint32_t kmax);
 

@@ -176,6 +176,103 @@ __device__ double score_lane(const uint8_t *col, const DevProblem &pr, uint32_t 
   return obj;
 }
 
+// Wave-local state of the tie-keeping filter: the L smallest distinct objectives this
+// wave has seen for the current job; its L-th bounds the job's global threshold.
+struct LevelFilter {
+  double lvl[kMaxLevels];
+  uint32_t n;
+};
+
+__device__ __forceinline__ void init_rows(const RunParams &p, uint8_t *wl, int lane) {
+  // dummy site N of every problem: padding entries of the neighbour stream read it
+  for (uint32_t q = 0; q < p.nprob; ++q)
+    wl[p.off_rows[q] + (p.prob[q].N << 6) + lane] =
+        static_cast<uint8_t>(p.prob[q].K * p.prob[q].F);
+}
+
+// One segment of a job: 64 lanes x p.rounds candidates starting at sbeg, candidates at or
+// beyond `end` are inactive.  Generates, scores, and appends candidates at or below the
+// job's threshold to its record sink.
+template <int KM>
+__device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, uint8_t *wl,
+                             uint32_t *hl, int lane, LevelFilter &lf,
+                             unsigned long long &scored) {
+  for (uint32_t r = 0; r < p.rounds; ++r) {
+    const uint64_t cand = p.systematic ? sbeg + static_cast<uint64_t>(lane) * p.rounds + r
+                                       : sbeg + static_cast<uint64_t>(r) * 64 + lane;
+    const bool active = cand < end;
+    // every lane keeps a valid configuration in its column (inactive lanes score the
+    // sorted one and discard it), so every LDS address below stays in the wave's region
+    for (uint32_t q = 0; q < p.nprob; ++q) {
+      const DevProblem &pr = p.prob[q];
+      uint8_t *col = wl + p.off_rows[q] + lane;
+      if (p.systematic) {
+        if (r == 0) {
+          load_col(col, pr);
+          if (active) unrank_col(col, pr, cand + 1);
+        } else if (active) {
+          next_perm_col(col, pr.bounds[0], pr.bounds[1] - pr.bounds[0]);
+        }
+      } else {
+        load_col(col, pr);
+        if (active && !(p.chunk_quirk && cand % p.chunk_size == 0)) shuffle_col(col, pr, cand);
+      }
+    }
+    if (__ballot(active) == 0) break;  // the tail of a systematic segment
+    double sub[kMaxProblems];
+    for (uint32_t q = 0; q < p.nprob; ++q)
+      sub[q] = score_lane<KM>(wl + p.off_rows[q] + lane, p.prob[q], hl, nullptr, nullptr);
+    double tot;
+    if (p.nprob == 1) {
+      tot = sub[0];
+    } else {  // helpers::sum folds from T(0) (core/helpers/fold.h:51-53)
+      tot = 0.0;
+      for (uint32_t q = 0; q < p.nprob; ++q) tot += sub[q];
+    }
+    scored += static_cast<unsigned long long>(__popcll(__ballot(active)));
+    const unsigned long long g =
+        __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool pass = active && okey(tot) <= g;
+    if (pass) {
+      const uint32_t slot = atomicAdd(p.rec_count, 1u);
+      if (slot < p.rec_cap) {
+        uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
+        uint32_t off = 0;
+        for (uint32_t q = 0; q < p.nprob; ++q) {
+          const DevProblem &pr = p.prob[q];
+          const uint8_t *col = wl + p.off_rows[q] + lane;
+          for (uint32_t i = 0; i < pr.N; ++i) dst[off + i] = col[i << 6] / pr.F;
+          off += pr.N;
+        }
+        p.rec_cand[slot] = cand;
+        p.rec_obj[slot] = tot;
+        for (uint32_t q = 0; q < p.nprob; ++q) p.rec_sub[slot * p.nprob + q] = sub[q];
+      }
+    }
+    unsigned long long m = __ballot(pass);
+    bool changed = false;
+    while (m) {
+      const int l = __ffsll(m) - 1;
+      m &= m - 1;
+      const double o = __shfl(tot, l);
+      bool present = false;
+      for (uint32_t i = 0; i < lf.n; ++i) present |= (lf.lvl[i] == o);
+      if (present || !(lf.n < p.L || o < lf.lvl[lf.n - 1])) continue;
+      uint32_t pos = lf.n < p.L ? lf.n++ : lf.n - 1;
+      while (pos > 0 && lf.lvl[pos - 1] > o) {
+        lf.lvl[pos] = lf.lvl[pos - 1];
+        --pos;
+      }
+      lf.lvl[pos] = o;
+      changed = true;
+    }
+    if (changed && lf.n == p.L && lane == 0) {
+      const unsigned long long k = okey(lf.lvl[p.L - 1]);
+      if (k < g) atomicMin(p.thr_key, k);
+    }
+  }
+}
+
 template <int KM>
 __global__ __launch_bounds__(256) void run_lane_kernel(RunParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -188,91 +285,61 @@ __global__ __launch_bounds__(256) void run_lane_kernel(RunParams p) {
   const uint64_t per_seg = 64ull * p.rounds;
   const uint64_t total = p.cand_end - p.cand_begin;
   const uint64_t nseg = (total + per_seg - 1) / per_seg;
-  double lvl[kMaxLevels];
-  uint32_t nlvl = 0;
+  LevelFilter lf;
+  lf.n = 0;
   unsigned long long scored = 0;
-  // dummy site N of every problem: padding entries of the neighbour stream read it
-  for (uint32_t q = 0; q < p.nprob; ++q)
-    wl[p.off_rows[q] + (p.prob[q].N << 6) + lane] = static_cast<uint8_t>(p.prob[q].K * p.prob[q].F);
-  for (uint64_t seg = gw; seg < nseg; seg += GW) {
-    const uint64_t sbeg = p.cand_begin + seg * per_seg;
-    for (uint32_t r = 0; r < p.rounds; ++r) {
-      const uint64_t cand = p.systematic ? sbeg + static_cast<uint64_t>(lane) * p.rounds + r
-                                         : sbeg + static_cast<uint64_t>(r) * 64 + lane;
-      const bool active = cand < p.cand_end;
-      // every lane keeps a valid configuration in its column (inactive lanes score the
-      // sorted one and discard it), so every LDS address below stays in the wave's region
-      for (uint32_t q = 0; q < p.nprob; ++q) {
-        const DevProblem &pr = p.prob[q];
-        uint8_t *col = wl + p.off_rows[q] + lane;
-        if (p.systematic) {
-          if (r == 0) {
-            load_col(col, pr);
-            if (active) unrank_col(col, pr, cand + 1);
-          } else if (active) {
-            next_perm_col(col, pr.bounds[0], pr.bounds[1] - pr.bounds[0]);
-          }
-        } else {
-          load_col(col, pr);
-          if (active && !(p.chunk_quirk && cand % p.chunk_size == 0)) shuffle_col(col, pr, cand);
-        }
-      }
-      if (__ballot(active) == 0) break;  // the tail of a systematic segment
-      double sub[kMaxProblems];
-      for (uint32_t q = 0; q < p.nprob; ++q)
-        sub[q] = score_lane<KM>(wl + p.off_rows[q] + lane, p.prob[q], hl, nullptr, nullptr);
-      double tot;
-      if (p.nprob == 1) {
-        tot = sub[0];
-      } else {  // helpers::sum folds from T(0) (core/helpers/fold.h:51-53)
-        tot = 0.0;
-        for (uint32_t q = 0; q < p.nprob; ++q) tot += sub[q];
-      }
-      scored += static_cast<unsigned long long>(__popcll(__ballot(active)));
-      const unsigned long long g =
-          __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const bool pass = active && okey(tot) <= g;
-      if (pass) {
-        const uint32_t slot = atomicAdd(p.rec_count, 1u);
-        if (slot < p.rec_cap) {
-          uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
-          uint32_t off = 0;
-          for (uint32_t q = 0; q < p.nprob; ++q) {
-            const DevProblem &pr = p.prob[q];
-            const uint8_t *col = wl + p.off_rows[q] + lane;
-            for (uint32_t i = 0; i < pr.N; ++i) dst[off + i] = col[i << 6] / pr.F;
-            off += pr.N;
-          }
-          p.rec_cand[slot] = cand;
-          p.rec_obj[slot] = tot;
-          for (uint32_t q = 0; q < p.nprob; ++q) p.rec_sub[slot * p.nprob + q] = sub[q];
-        }
-      }
-      // wave-uniform bookkeeping of the L smallest distinct objectives seen by this wave
-      unsigned long long m = __ballot(pass);
-      bool changed = false;
-      while (m) {
-        const int l = __ffsll(m) - 1;
-        m &= m - 1;
-        const double o = __shfl(tot, l);
-        bool present = false;
-        for (uint32_t i = 0; i < nlvl; ++i) present |= (lvl[i] == o);
-        if (present || !(nlvl < p.L || o < lvl[nlvl - 1])) continue;
-        uint32_t pos = nlvl < p.L ? nlvl++ : nlvl - 1;
-        while (pos > 0 && lvl[pos - 1] > o) {
-          lvl[pos] = lvl[pos - 1];
-          --pos;
-        }
-        lvl[pos] = o;
-        changed = true;
-      }
-      if (changed && nlvl == p.L && lane == 0) {
-        const unsigned long long k = okey(lvl[p.L - 1]);
-        if (k < g) atomicMin(p.thr_key, k);
-      }
-    }
-  }
+  init_rows(p, wl, lane);
+  for (uint64_t seg = gw; seg < nseg; seg += GW)
+    lane_segment<KM>(p, p.cand_begin + seg * per_seg, p.cand_end, wl, hl, lane, lf, scored);
   if (lane == 0 && scored) atomicAdd(p.scored, scored);
+}
+
+// Persistent form: every wavefront pulls work items (job, candidate range) from a shared
+// counter until the list is exhausted; jobs of any size, species and shell count (up to
+// KM species) share the launch.  Each job has its own threshold and record sink.
+template <int KM>
+__global__ __launch_bounds__(256) void batch_lane_kernel(BatchParams bp) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = threadIdx.x >> 6;
+  uint8_t *wl = lds + wave * bp.wave_lds;
+  const uint32_t num_items = *bp.num_items;
+  uint32_t last_job = 0xffffffffu;
+  LevelFilter lf;
+  lf.n = 0;
+  unsigned long long scored = 0;
+  for (;;) {
+    uint32_t item = 0;
+    if (lane == 0) item = atomicAdd(bp.next_item, 1u);
+    item = __shfl(item, 0);
+    if (item >= num_items) break;  // every wave leaves once the list is drained
+    const WorkItem wi = bp.items[item];
+    const RunParams &p = bp.jobs[wi.job];
+    if (wi.job != last_job) {
+      if (last_job != 0xffffffffu && lane == 0 && scored)
+        atomicAdd(bp.jobs[last_job].scored, scored);
+      scored = 0;
+      lf.n = 0;
+      last_job = wi.job;
+      init_rows(p, wl, lane);
+    }
+    uint32_t *hl = reinterpret_cast<uint32_t *>(wl + p.off_hist) + lane;
+    const uint64_t per_seg = 64ull * p.rounds;
+    for (uint64_t sbeg = wi.begin; sbeg < wi.end; sbeg += per_seg)
+      lane_segment<KM>(p, sbeg, wi.end, wl, hl, lane, lf, scored);
+  }
+  if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
+}
+
+// zeroes the item counter and every job's record and scored counters (a kernel, so the
+// reset can sit inside a captured graph)
+__global__ void batch_reset_kernel(BatchParams bp, uint32_t max_jobs) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *bp.next_item = 0u;
+  if (t < max_jobs && t < *bp.num_jobs) {
+    *bp.jobs[t].rec_count = 0u;
+    *bp.jobs[t].scored = 0ull;
+  }
 }
 
 template <int KM>
@@ -322,7 +389,57 @@ template <int KM> hipError_t launch_score_lane_k(const ScoreParams &p, hipStream
   return hipGetLastError();
 }
 
+template <int KM> hipError_t launch_batch_lane_k(const BatchParams &bp, uint32_t blocks,
+                                                 uint32_t W, hipStream_t s) {
+  const size_t lds = static_cast<size_t>(W) * bp.wave_lds;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_lane_kernel<KM>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(batch_lane_kernel<KM>, dim3(blocks), dim3(64 * W), lds, s, bp);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_batch_reset(const BatchParams &bp, uint32_t max_jobs, hipStream_t s) {
+  const uint32_t threads = 256, blocks = (max_jobs + threads) / threads;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(blocks), dim3(threads), 0, s, bp, max_jobs);
+  return hipGetLastError();
+}
+
+hipError_t prepare_batch_lane(uint32_t kmax, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  const void *fn = nullptr;
+  switch (kmax) {
+    case 1: case 2: fn = reinterpret_cast<const void *>(&batch_lane_kernel<2>); break;
+    case 3: fn = reinterpret_cast<const void *>(&batch_lane_kernel<3>); break;
+    case 4: fn = reinterpret_cast<const void *>(&batch_lane_kernel<4>); break;
+    case 5: fn = reinterpret_cast<const void *>(&batch_lane_kernel<5>); break;
+    case 6: fn = reinterpret_cast<const void *>(&batch_lane_kernel<6>); break;
+    case 7: fn = reinterpret_cast<const void *>(&batch_lane_kernel<7>); break;
+    case 8: fn = reinterpret_cast<const void *>(&batch_lane_kernel<8>); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(lds));
+}
+
+hipError_t launch_batch_lane(const BatchParams &bp, uint32_t kmax, uint32_t blocks,
+                             uint32_t waves_per_block, hipStream_t s) {
+  switch (kmax) {
+    case 1: case 2: return launch_batch_lane_k<2>(bp, blocks, waves_per_block, s);
+    case 3: return launch_batch_lane_k<3>(bp, blocks, waves_per_block, s);
+    case 4: return launch_batch_lane_k<4>(bp, blocks, waves_per_block, s);
+    case 5: return launch_batch_lane_k<5>(bp, blocks, waves_per_block, s);
+    case 6: return launch_batch_lane_k<6>(bp, blocks, waves_per_block, s);
+    case 7: return launch_batch_lane_k<7>(bp, blocks, waves_per_block, s);
+    case 8: return launch_batch_lane_k<8>(bp, blocks, waves_per_block, s);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 uint32_t lane_field_bits(uint32_t kmax) { return lane_bits(kmax < 2 ? 2 : static_cast<int>(kmax)); }
 

@@ -259,3 +259,86 @@ def test_systematic_32_optimal_rank_set(gpu):
     ranks = np.sort(pack.arrays(0, n_best, species=False, ranks=True)["rank"])
     assert ranks[:64].tolist() == golden["best_ranks_head"]
     assert hashlib.sha256(ranks.astype("<u8").tobytes()).hexdigest() == golden["best_ranks_sha256"]
+
+
+def _small(name, iterations=131072):
+    d = WORKLOADS[name]()
+    d["iterations"] = iterations
+    d["chunk_size"] = min(iterations, 100000)
+    return d
+
+
+BATCH_MIX = ["fcc108_ternary", "bcc128_quaternary", "hcp128_binary", "rocksalt216_split",
+             "fcc256_quinary", "fcc32_3shells"]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_matches_single_job_runs(gpu, use_graph):
+    """uneven jobs (108/128/128/216/256/32 sites, 2-5 species, 3-7 shells) in shared
+    persistent launches return exactly the packs of their single-job runs"""
+    print(f"[batch] graph={use_graph}", flush=True)
+    alone = []
+    for name in BATCH_MIX:
+        job = Job(parse_config(_small(name)))
+        job.set_kernel(2)
+        job.run()
+        alone.append(_pack_signature(job.results()))
+        job.close()
+    jobs = [Job(parse_config(_small(name))) for name in BATCH_MIX]
+    for j in jobs:
+        j.set_launch_size(32768)  # several persistent steps per job
+    rep = S.batch_run(jobs, use_graph=use_graph)
+    assert rep["candidates"] == 131072 * len(jobs)
+    assert rep["launches"] >= 4
+    for j, ref in zip(jobs, alone):
+        assert j.timing()[2] == 131072
+        assert _pack_signature(j.results()) == ref
+        j.close()
+
+
+def test_batch_systematic_all_results(gpu):
+    d = W.reference_test_16()
+    job = Job(parse_config(d))
+    job.set_launch_size(2000)
+    S.batch_run([job])
+    pack = job.results()
+    assert pack.num_results() == 12870
+    ranks = np.sort(pack.arrays(species=False)["rank"])
+    assert np.array_equal(ranks, np.arange(1, 12871))
+
+
+def test_server_results_alone_and_in_the_mix(gpu):
+    """continuous batching: jobs admitted while others are resident produce the same packs
+    as when run alone"""
+    names = ["fcc256_quinary", "fcc32_3shells", "hcp128_binary", "fcc108_ternary",
+             "rocksalt216_split", "bcc128_quaternary"]
+    its = [65536, 200000, 100000, 70000, 50000, 90000]
+    alone = []
+    for name, it in zip(names, its):
+        job = Job(parse_config(_small(name, it)))
+        job.set_kernel(2)
+        job.run()
+        alone.append(_pack_signature(job.results()))
+        job.close()
+    srv = S.Server(max_active=3, slice_candidates=16384)
+    jobs = [Job(parse_config(_small(n, it))) for n, it in zip(names, its)]
+    tickets = [srv.submit(jobs[0]), srv.submit(jobs[1])]
+    srv.step()
+    tickets += [srv.submit(j) for j in jobs[2:4]]
+    srv.step()
+    srv.step()
+    tickets += [srv.submit(j) for j in jobs[4:]]
+    srv.drain()
+    for t, j, ref in zip(tickets, jobs, alone):
+        st = srv.poll(t)
+        assert st["done"] and st["t_admit"] >= st["t_submit"] and st["t_done"] >= st["t_admit"]
+        assert _pack_signature(j.results()) == ref
+    srv.close()
+
+
+def test_autotune_picks_a_variant(gpu):
+    job = Job(parse_config(_small("fcc256_quinary")))
+    kernel, rate = job.autotune()
+    assert kernel in (1, 2) and rate > 0
+    job.run()
+    assert job.timing()[2] == 131072
