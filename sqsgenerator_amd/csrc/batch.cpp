@@ -3,8 +3,9 @@
 // BatchEngine keeps a set of resident jobs (their inputs, thresholds and record sinks stay
 // in device memory) and advances all of them with one persistent launch per step: each
 // step gives every active job its next slice of candidates, split into work items that the
-// persistent wavefronts pull from a shared counter.  Jobs are grouped by their largest
-// species count (the lane kernel is specialised on it); each group has fixed device tables
+// persistent wavefronts pull from a shared counter.  Jobs are grouped by kernel variant and
+// its specialisation (largest species count for the lane kernel, tile bound for the dense
+// MFMA kernel); each group has fixed device tables
 // and a captured graph {counter reset kernel, persistent kernel} that is replayed every
 // step.  Between steps the host drains the record sinks, tightens each job's threshold and
 // admits or retires jobs, which is what the serving loop builds on.
@@ -26,8 +27,11 @@ namespace {
 constexpr uint32_t kGroupJobs = 256;      // resident jobs per species-count group
 constexpr uint32_t kGroupItems = 1u << 17;
 
+struct Group;
+
 struct Slot {
   sqsg_job *job = nullptr;
+  Group *group = nullptr;
   RunParams p{};
   RecordSink sink;
   DevBuf thr, count, scored;
@@ -40,7 +44,8 @@ struct Slot {
 };
 
 struct Group {
-  uint32_t kmax = 0;
+  int variant = 0;
+  uint32_t key = 0;  // lane: largest species count; dense: tile bound
   DevBuf d_jobs, d_items, d_counts, d_next;
   hipGraphExec_t exec = nullptr;
   uint32_t graph_wave_lds = 0, graph_blocks = 0;
@@ -53,7 +58,8 @@ struct Group {
 size_t job_footprint(const sqsg_job *job) {
   size_t b = 0;
   for (const auto &P : job->problems)
-    b += P.pi.size() * 4 + P.lane_stream.size() * 4 + P.pref.size() * 24 + 2 * P.N + 1024;
+    b += P.pi.size() * 4 + P.lane_stream.size() * 4 + P.pref.size() * 24 + 2 * P.N + 1024 +
+         size_t(P.dtiles_n) * 1024 + P.dmask.size() * 4 + 32 * P.dT;
   return b;
 }
 
@@ -76,24 +82,27 @@ class BatchEngine {
   // Makes a job resident; its run state is reset exactly as sqsg_run would.
   size_t add(sqsg_job *job) {
     begin_run(job);
-    if (resolve_kernel(job, job->kernel == SQSG_KERNEL_SPARSE_WAVE ? SQSG_KERNEL_AUTO
-                                                                   : job->kernel) !=
-        SQSG_KERNEL_SPARSE_LANE)
-      throw std::invalid_argument("batched execution needs a job the lane kernel handles");
+    const int variant = resolve_kernel(job, job->kernel == SQSG_KERNEL_SPARSE_WAVE
+                                                ? SQSG_KERNEL_AUTO : job->kernel);
+    if (variant == SQSG_KERNEL_SPARSE_WAVE)
+      throw std::invalid_argument("batched execution needs a job the lane or dense kernel "
+                                  "handles");
+    if (variant == SQSG_KERNEL_DENSE_MFMA) ensure_dense(job);
     auto s = std::make_unique<Slot>();
     s->job = job;
     uint32_t blocks = 0;
-    s->p = make_run_params(job, SQSG_KERNEL_SPARSE_LANE, 1, &blocks);
+    s->p = make_run_params(job, variant, 1, &blocks);
     s->slice = job->launch_size ? job->launch_size : default_launch_size(job);
     uint64_t work = 0;
     for (const auto &P : job->problems) work += P.pi.size() + 64;
+    const uint64_t gen = s->p.gen_group;
     if (s->p.systematic) {
       s->p.rounds = static_cast<uint32_t>(std::clamp<uint64_t>(
-          (uint64_t(1) << 21) / (64 * std::max<uint64_t>(work, 1)), 1, 256));
+          (uint64_t(1) << 21) / (gen * std::max<uint64_t>(work, 1)), 1, 256));
     } else {
       s->p.rounds = 1;
     }
-    const uint64_t seg = 64ull * s->p.rounds;
+    const uint64_t seg = gen * s->p.rounds;
     s->item = std::max<uint64_t>(seg, ((uint64_t(1) << 22) / std::max<uint64_t>(work, 1)) / seg * seg);
     s->slice = std::max(s->slice, s->item);
     s->p.rec_stride = job->stride;
@@ -108,9 +117,11 @@ class BatchEngine {
     s->p.rec_count = s->count.as<unsigned int>();
     s->p.scored = s->scored.as<unsigned long long>();
     s->footprint = job_footprint(job) + s->sink.cap * (24 + job->stride);
-    Group &g = group(job->kmax);
+    Group &g = group(variant, variant == SQSG_KERNEL_DENSE_MFMA ? dense_tile_bound_of(job)
+                                                                : job->kmax);
     if (g.members.size() >= kGroupJobs) throw std::out_of_range("too many resident jobs");
     g.members.push_back(s.get());
+    s->group = &g;
     const size_t fp = s->footprint;
     slots_.push_back(std::move(s));
     return fp;
@@ -179,7 +190,7 @@ class BatchEngine {
       job->stats.timing_loop_ns +=
           std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
       job->stats.timing_total_ns = job->stats.timing_loop_ns;
-      Group &g = group(job->kmax);
+      Group &g = *(*it)->group;
       g.members.erase(std::find(g.members.begin(), g.members.end(), it->get()));
       finished.push_back(job);
       it = slots_.erase(it);
@@ -195,11 +206,12 @@ class BatchEngine {
   uint32_t last_items = 0;
 
  private:
-  Group &group(uint32_t kmax) {
-    auto &g = groups_[kmax];
+  Group &group(int variant, uint32_t key) {
+    auto &g = groups_[(static_cast<uint32_t>(variant) << 8) | key];
     if (!g) {
       g = std::make_unique<Group>();
-      g->kmax = kmax;
+      g->variant = variant;
+      g->key = key;
       g->d_jobs.alloc(kGroupJobs * sizeof(RunParams));
       g->d_items.alloc(kGroupItems * sizeof(WorkItem));
       g->d_counts.alloc(2 * sizeof(uint32_t));
@@ -236,8 +248,10 @@ class BatchEngine {
                             stream_));
     SQSG_HIP(hipStreamSynchronize(stream_));  // `table`/`items` are stack buffers
     const size_t block_lds = wave_lds;
+    const bool dense = g.variant == SQSG_KERNEL_DENSE_MFMA;
     uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / std::max<size_t>(block_lds, 1));
-    per_cu = std::max(1u, std::min(per_cu, 16u));
+    // dense wavefronts fill a SIMD's register file: one per SIMD
+    per_cu = std::max(1u, std::min(per_cu, dense ? 4u : 16u));
     const uint32_t blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
     BatchParams bp{};
     bp.jobs = g.d_jobs.as<const RunParams>();
@@ -246,10 +260,14 @@ class BatchEngine {
     bp.num_jobs = g.d_counts.as<const uint32_t>() + 1;
     bp.wave_lds = wave_lds;
     bp.next_item = g.d_next.as<unsigned int>();
-    SQSG_HIP(prepare_batch_lane(g.kmax, block_lds));
+    SQSG_HIP(dense ? prepare_batch_dense(g.key, block_lds) : prepare_batch_lane(g.key, block_lds));
+    auto launch = [&]() {
+      return dense ? launch_batch_dense(bp, g.key, blocks, stream_)
+                   : launch_batch_lane(bp, g.key, blocks, 1, stream_);
+    };
     if (!use_graph_) {
       SQSG_HIP(launch_batch_reset(bp, kGroupJobs, stream_));
-      SQSG_HIP(launch_batch_lane(bp, g.kmax, blocks, 1, stream_));
+      SQSG_HIP(launch());
       return;
     }
     if (!g.exec || g.graph_wave_lds != wave_lds || g.graph_blocks != blocks) {
@@ -259,7 +277,7 @@ class BatchEngine {
       hipGraph_t graph = nullptr;
       SQSG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
       const hipError_t e1 = launch_batch_reset(bp, kGroupJobs, stream_);
-      const hipError_t e2 = launch_batch_lane(bp, g.kmax, blocks, 1, stream_);
+      const hipError_t e2 = launch();
       SQSG_HIP(hipStreamEndCapture(stream_, &graph));
       SQSG_HIP(e1);
       SQSG_HIP(e2);
@@ -301,6 +319,7 @@ void autotune(sqsg_job *job, int *chosen, double *rate) {
   ensure_device(job);
   std::vector<int> variants{SQSG_KERNEL_SPARSE_WAVE};
   if (job->lane_ok) variants.push_back(SQSG_KERNEL_SPARSE_LANE);
+  if (job->dense_ok) variants.push_back(SQSG_KERNEL_DENSE_MFMA);
   const uint64_t saved_it = job->iterations, saved_launch = job->launch_size;
   const int saved_kernel = job->kernel;
   uint64_t work = 0;

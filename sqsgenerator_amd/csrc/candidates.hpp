@@ -1,0 +1,119 @@
+// Candidate generation and tie-keeping helpers shared by the scoring kernels.
+//
+// A candidate's species live in LDS as N bytes with a fixed stride: contiguous rows
+// (stride 1: the wave and dense kernels) or one column of a [site][64 lanes] block
+// (stride 64: the lane kernel).  The generators below are the device definition of a
+// candidate, identical for every kernel:
+//  * MC candidate t: Fisher-Yates of the sorted configuration with draws
+//    [t * draws, (t + 1) * draws) of the rapidrand stream (core/shuffle.h:23-50; the stream
+//    is a counter, state_k = seed + k * s0), windows in bounds order, i from w down to 2;
+//  * systematic rank r: the 1-based lexicographic unrank of the window bounds[0..1]
+//    (core/permutation.h:52-78) and its successors (core/permutation.h:80-102).
+#pragma once
+
+#include "device_common.hpp"
+#include "device_params.hpp"
+
+namespace sqsg {
+namespace {
+
+template <uint32_t SH> struct Strided {
+  uint8_t *p;
+  __device__ __forceinline__ uint8_t &operator[](uint32_t i) const { return p[i << SH]; }
+};
+
+template <uint32_t SH>
+__device__ __forceinline__ void shuffle_candidate(Strided<SH> v, const DevProblem &pr,
+                                                  uint64_t cand) {
+  uint64_t st = pr.seed + (cand * static_cast<uint64_t>(pr.draws)) * kSecret0;
+  for (uint32_t b = 0; b < pr.nbounds; ++b) {
+    const uint32_t lo = pr.bounds[2 * b], hi = pr.bounds[2 * b + 1];
+    for (uint32_t i = hi - lo; i > 1; --i) {
+      const uint32_t p = static_cast<uint32_t>((static_cast<uint64_t>(draw32(st)) * i) >> 32);
+      const uint8_t t = v[lo + i - 1];
+      v[lo + i - 1] = v[lo + p];
+      v[lo + p] = t;
+    }
+  }
+}
+
+// writes the packed species of rank `rank` (1-based) times `scale` into the window
+template <uint32_t SH>
+__device__ void unrank_candidate(Strided<SH> v, const DevProblem &pr, uint64_t rank,
+                                 uint32_t scale) {
+  uint32_t hist[kMaxLocalSpecies];
+  for (uint32_t j = 0; j < kMaxLocalSpecies; ++j) hist[j] = pr.sys_hist[j];
+  const uint32_t lo = pr.bounds[0], w = pr.bounds[1] - pr.bounds[0];
+  uint64_t total = 1;
+  uint32_t placed = 0;
+  for (uint32_t j = 0; j < pr.sys_nspecies; ++j)
+    for (uint32_t k = 1; k <= hist[j]; ++k) total = muldiv(total, ++placed, k);
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < w; ++i)
+    for (uint32_t j = 0; j < pr.sys_nspecies; ++j) {
+      const uint64_t sc = muldiv(total, hist[j], w - i);
+      if (rank <= sc) {
+        v[lo + k++] = static_cast<uint8_t>(pr.sys_rev[j] * scale);
+        total = sc;
+        --hist[j];
+        break;
+      }
+      rank -= sc;
+    }
+}
+
+template <uint32_t SH>
+__device__ void next_perm_candidate(Strided<SH> v, uint32_t lo, uint32_t w) {
+  if (w < 2) return;
+  int k = static_cast<int>(w) - 2;
+  while (k >= 0 && v[lo + k] >= v[lo + k + 1]) --k;
+  if (k < 0) return;
+  int l = static_cast<int>(w) - 1;
+  while (v[lo + k] >= v[lo + l]) --l;
+  uint8_t t = v[lo + k];
+  v[lo + k] = v[lo + l];
+  v[lo + l] = t;
+  for (int a = k + 1, b = static_cast<int>(w) - 1; a < b; ++a, --b) {
+    t = v[lo + a];
+    v[lo + a] = v[lo + b];
+    v[lo + b] = t;
+  }
+}
+
+// Wave-local state of the tie-keeping filter: the L smallest distinct objectives this
+// wave has seen for the current job; its L-th bounds the job's global threshold.
+struct LevelFilter {
+  double lvl[kMaxLevels];
+  uint32_t n;
+};
+
+// Offers the objectives of the lanes in `pass` (a ballot) to the filter and lowers the
+// job's threshold when the wave holds L levels below it.  `g` is the threshold read
+// before the candidates were tested.
+__device__ __forceinline__ void offer_levels(LevelFilter &lf, const RunParams &p,
+                                             unsigned long long pass, double tot,
+                                             unsigned long long g, int lane) {
+  bool changed = false;
+  while (pass) {
+    const int l = __ffsll(pass) - 1;
+    pass &= pass - 1;
+    const double o = __shfl(tot, l);
+    bool present = false;
+    for (uint32_t i = 0; i < lf.n; ++i) present |= (lf.lvl[i] == o);
+    if (present || !(lf.n < p.L || o < lf.lvl[lf.n - 1])) continue;
+    uint32_t pos = lf.n < p.L ? lf.n++ : lf.n - 1;
+    while (pos > 0 && lf.lvl[pos - 1] > o) {
+      lf.lvl[pos] = lf.lvl[pos - 1];
+      --pos;
+    }
+    lf.lvl[pos] = o;
+    changed = true;
+  }
+  if (changed && lf.n == p.L && lane == 0) {
+    const unsigned long long k = okey(lf.lvl[p.L - 1]);
+    if (k < g) atomicMin(p.thr_key, k);
+  }
+}
+
+}  // namespace
+}  // namespace sqsg

@@ -20,9 +20,14 @@ struct DevProblem {
   const double *pw;           // [S][K][K] scaled by the shell weights
   const double *tgt;          // [S][K][K]
   const uint32_t *terms;      // T objective terms in reference order: s | xi << 8 | eta << 16
-  const uint32_t *nbr_start;  // N + 1 CSR offsets of the upper neighbour lists
-  const uint32_t *nbr;        // P entries j | shell << 16 (per site i, j > i)
-  const int8_t *adj;          // dense: [S][Npad][Npad] upper-triangular 0/1 (transposed: [s][j][i])
+  // dense contraction: 32 x 32 int8 tiles of the upper shell adjacency in the A-operand
+  // layout of v_mfma_i32_32x32x32_i8, streamed in (shell, row tile, column tile) order;
+  // dmask[s * dT + I] has bit J set when tile (s, I, J) is in the stream
+  const uint32_t *dtiles;     // 256 words per tile
+  const uint32_t *dmask;
+  const uint8_t *species0D;   // sorted configuration, 32 * dT bytes, padding sites 0xff
+  uint32_t dT;                // 32-site tiles
+  uint32_t dtiles_n;
   // lane-per-candidate layout: per shell s, groups [i | count << 16] followed by `count`
   // upper neighbours j of site i; lane_off[s] .. lane_off[s + 1] delimit shell s
   const uint32_t *lane_stream;
@@ -30,7 +35,6 @@ struct DevProblem {
   const uint8_t *species0F;   // species0 * F (field shift of each species), 4-byte padded
   uint32_t F;                 // bits per packed neighbour-count field (K * F <= 32)
   uint32_t N, K, S, P, T;
-  uint32_t Npad;              // N rounded up to the dense tile (32)
   uint32_t nbounds;
   uint32_t bounds[2 * kMaxBounds];
   uint32_t draws;             // RNG draws per candidate: sum over windows of (w - 1)
@@ -52,10 +56,12 @@ struct RunParams {
   uint64_t cand_begin, cand_end;
   uint32_t L;                  // distinct objective levels kept (keep + 1)
   uint32_t gen_group;          // candidates generated per wave and round (<= 64)
+  uint32_t dense_cw;           // dense: candidates per MFMA pass (G groups of 32 / K)
+  uint32_t dense_stride;       // dense: bytes of one LDS candidate row
   // LDS layout of one wave (bytes)
   uint32_t wave_lds;
   uint32_t off_rows[kMaxProblems];
-  uint32_t off_hist, off_terms, off_obj;
+  uint32_t off_hist, off_terms, off_obj;  // dense: off_hist holds one shell's counts
   // tie-keeping threshold and record sink
   unsigned long long *thr_key;
   unsigned int *rec_count;
@@ -92,6 +98,7 @@ struct ScoreParams {
   double *objective;           // B
   double *sro;                 // B x S x K x K or null
   uint32_t wave_lds, off_row, off_hist, off_terms;
+  uint32_t dense_cw, dense_gen, dense_stride;
 };
 
 }  // namespace sqsg

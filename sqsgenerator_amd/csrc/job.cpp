@@ -210,6 +210,61 @@ void Problem::build_lane(uint32_t field_bits) {
   upload(d_species0F, sF, 16);
 }
 
+// Tiles in the A-operand layout of v_mfma_i32_32x32x32_i8: lane l, byte q of tile (s, I, J)
+// is U_s[32I + perm(l % 32)][32J + 16(l / 32) + q] with perm(m) = 16((m / 4) % 2) +
+// 4(m / 8) + m % 4 (the accumulator row order of the 32 x 32 result).  Tiles without a
+// pair are left out of the stream and their bit in dmask is clear.
+void Problem::build_dense() {
+  if (d_dtiles.size()) return;
+  dT = (N + 31) / 32;
+  const uint32_t n32 = 32 * dT;
+  std::vector<uint8_t> sh(size_t(n32) * n32, 0xff);
+  for (size_t q = 0; q < pi.size(); ++q) sh[size_t(pi[q]) * n32 + pj[q]] = static_cast<uint8_t>(ps[q]);
+  dmask.assign(size_t(S) * dT, 0u);
+  std::vector<uint8_t> tiles;
+  for (uint32_t s = 0; s < S; ++s)
+    for (uint32_t I = 0; I < dT; ++I)
+      for (uint32_t J = I; J < dT; ++J) {
+        bool any = false;
+        for (uint32_t i = 32 * I; i < 32 * I + 32 && !any; ++i)
+          for (uint32_t j = 32 * J; j < 32 * J + 32; ++j)
+            if (sh[size_t(i) * n32 + j] == s) {
+              any = true;
+              break;
+            }
+        if (!any) continue;
+        dmask[size_t(s) * dT + I] |= 1u << J;
+        const size_t base = tiles.size();
+        tiles.resize(base + 1024, 0);
+        for (uint32_t l = 0; l < 64; ++l) {
+          const uint32_t m = l % 32;
+          const uint32_t i = 32 * I + 16 * ((m / 4) % 2) + 4 * (m / 8) + m % 4;
+          for (uint32_t q = 0; q < 16; ++q) {
+            const uint32_t j = 32 * J + 16 * (l / 32) + q;
+            tiles[base + 16 * l + q] = sh[size_t(i) * n32 + j] == s ? 1 : 0;
+          }
+        }
+      }
+  dtiles_n = static_cast<uint32_t>(tiles.size() / 1024);
+  std::vector<uint8_t> sp(n32, 0xff);
+  std::copy(species_packed.begin(), species_packed.end(), sp.begin());
+  upload(d_dtiles, tiles, 16);
+  upload(d_dmask, dmask);
+  upload(d_species0D, sp, 16);
+}
+
+void ensure_dense(sqsg_job *job) {
+  ensure_device(job);
+  if (!job->dense_ok)
+    throw std::invalid_argument("the dense kernel needs one sublattice problem of at most 1024 "
+                                "sites, 16 species and 127 upper neighbours per site and shell");
+  for (auto &P : job->problems) P.build_dense();
+}
+
+uint32_t dense_tile_bound_of(const sqsg_job *job) {
+  return dense_tile_bound((job->problems[0].N + 31) / 32);
+}
+
 void ensure_device(sqsg_job *job) {
   if (job->on_device) return;
   for (auto &P : job->problems) P.upload_device();
@@ -225,6 +280,14 @@ void ensure_device(sqsg_job *job) {
   }
   if (job->lane_ok)
     for (auto &P : job->problems) P.build_lane(lane_field_bits(job->kmax));
+  for (auto &P : job->problems) {
+    std::vector<uint32_t> deg(size_t(P.S) * P.N, 0);
+    P.max_upper = 0;
+    for (size_t q = 0; q < P.pi.size(); ++q)
+      P.max_upper = std::max(P.max_upper, ++deg[size_t(P.ps[q]) * P.N + P.pi[q]]);
+  }
+  job->dense_ok = job->problems.size() == 1 && job->problems[0].N <= 1024 &&
+                  job->problems[0].K <= 16 && job->problems[0].max_upper <= 127;
   SQSG_HIP(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking));
   SQSG_HIP(hipEventCreate(&job->ev0));
   SQSG_HIP(hipEventCreate(&job->ev1));
@@ -243,13 +306,14 @@ DevProblem Problem::dev() const {
   d.lane_off = d_lane_off.as<const uint32_t>();
   d.species0F = d_species0F.as<const uint8_t>();
   d.F = F;
-  d.nbr_start = d_nbr_start.as<const uint32_t>();
-  d.nbr = d_nbr.as<const uint32_t>();
-  d.adj = d_adj.as<const int8_t>();
+  d.dtiles = d_dtiles.as<const uint32_t>();
+  d.dmask = d_dmask.as<const uint32_t>();
+  d.species0D = d_species0D.as<const uint8_t>();
+  d.dT = dT;
+  d.dtiles_n = dtiles_n;
   d.N = N, d.K = K, d.S = S;
   d.P = static_cast<uint32_t>(pi.size());
   d.T = static_cast<uint32_t>(terms.size());
-  d.Npad = Npad;
   d.nbounds = static_cast<uint32_t>(bounds.size() / 2);
   for (size_t b = 0; b < bounds.size(); ++b) d.bounds[b] = bounds[b];
   d.draws = draws;
@@ -411,6 +475,12 @@ int resolve_kernel(const sqsg_job *job, int requested) {
         throw std::invalid_argument("the lane kernel needs at most 8 species and fewer than "
                                     "65536 pairs per shell");
       return requested;
+    case SQSG_KERNEL_DENSE_MFMA:
+      if (!job->dense_ok)
+        throw std::invalid_argument("the dense kernel needs one sublattice problem of at most "
+                                    "1024 sites, 16 species and 127 upper neighbours per site "
+                                    "and shell");
+      return requested;
     default:
       throw std::invalid_argument("kernel variant not available for this job");
   }
@@ -433,6 +503,25 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   p.L = static_cast<uint32_t>(job->keep + 1);
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   uint32_t off = 0;
+  if (variant == SQSG_KERNEL_DENSE_MFMA) {
+    // rows of gen_group candidates (npass passes of dense_cw) + one shell's counts
+    const Problem &P = job->problems[0];
+    const uint32_t T = (P.N + 31) / 32, G = dense_groups(dense_tile_bound(T));
+    const uint32_t g = std::min(32 / P.K, 64 / G);
+    p.dense_cw = G * g;
+    p.dense_stride = 32 * T + 16;
+    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4);
+    uint32_t npass = 64 / p.dense_cw;
+    while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 40 * 1024) --npass;
+    p.gen_group = npass * p.dense_cw;
+    p.off_rows[0] = 0;
+    p.off_hist = align(p.gen_group * p.dense_stride);
+    p.wave_lds = p.off_hist + cnt_bytes;
+    // one wavefront per block; the register file holds one dense wavefront per SIMD
+    const uint32_t per_cu = std::max(1u, std::min(4u, static_cast<uint32_t>((160 * 1024) / p.wave_lds)));
+    *blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
+    return p;
+  }
   if (variant == SQSG_KERNEL_SPARSE_LANE) {
     p.gen_group = 64;
     for (uint32_t q = 0; q < p.nprob; ++q) {
@@ -468,6 +557,8 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
 hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, uint32_t blocks,
                           uint32_t W, hipStream_t s) {
   if (variant == SQSG_KERNEL_SPARSE_LANE) return launch_run_lane(p, job->kmax, blocks, W, s);
+  if (variant == SQSG_KERNEL_DENSE_MFMA)
+    return launch_run_dense(p, dense_tile_bound_of(job), blocks, s);
   return launch_run_wave(p, blocks, W, s);
 }
 
@@ -534,7 +625,8 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   const auto t_start = clock::now();
   begin_run(job);
   const int variant = resolve_kernel(job, job->kernel);
-  const uint32_t W = variant == SQSG_KERNEL_SPARSE_LANE ? 1 : 4;
+  if (variant == SQSG_KERNEL_DENSE_MFMA) ensure_dense(job);
+  const uint32_t W = variant == SQSG_KERNEL_SPARSE_WAVE ? 4 : 1;
   uint32_t blocks = 0;
   RunParams p = make_run_params(job, variant, W, &blocks);
   const uint64_t launch = job->launch_size ? job->launch_size : default_launch_size(job);
@@ -631,6 +723,20 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     sp.wave_lds = align(sp.off_hist + job->kmax * ((job->kmax + 1) / 2) * 256);
     if (sp.wave_lds > 160 * 1024) throw std::out_of_range("problem exceeds LDS");
     SQSG_HIP(launch_score_lane(sp, job->kmax, s ? s : job->stream));
+    return;
+  }
+  if (variant == SQSG_KERNEL_DENSE_MFMA) {
+    ensure_dense(job);
+    uint32_t blocks = 0;
+    const RunParams rp = make_run_params(job, SQSG_KERNEL_DENSE_MFMA, 1, &blocks);
+    sp.prob = P.dev();
+    sp.dense_cw = rp.dense_cw;
+    sp.dense_gen = rp.gen_group;
+    sp.dense_stride = rp.dense_stride;
+    sp.off_row = rp.off_rows[0];
+    sp.off_hist = rp.off_hist;
+    sp.wave_lds = rp.wave_lds;
+    SQSG_HIP(launch_score_dense(sp, dense_tile_bound_of(job), s ? s : job->stream));
     return;
   }
   if (variant != SQSG_KERNEL_SPARSE_WAVE)

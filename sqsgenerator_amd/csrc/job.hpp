@@ -73,13 +73,19 @@ struct Problem {
   uint32_t sys_hist[kMaxLocalSpecies] = {};
   uint8_t sys_rev[kMaxLocalSpecies] = {};
   // device copies
-  DevBuf d_pairs, d_species0, d_pref, d_pw, d_tgt, d_terms, d_nbr_start, d_nbr, d_adj;
+  DevBuf d_pairs, d_species0, d_pref, d_pw, d_tgt, d_terms;
   // lane-per-candidate layout (built for the job's field width F)
   std::vector<uint32_t> lane_stream, lane_off;
   uint32_t F = 0;
   DevBuf d_lane_stream, d_lane_off, d_species0F;
   void build_lane(uint32_t field_bits);
-  uint32_t Npad = 0;
+  // dense contraction: 32-site tiles of the upper shell adjacency (built on first use)
+  uint32_t dT = 0;          // 32-site tiles
+  uint32_t max_upper = 0;   // largest count of upper neighbours of one site in one shell
+  uint32_t dtiles_n = 0;
+  std::vector<uint32_t> dmask;
+  DevBuf d_dtiles, d_dmask, d_species0D;
+  void build_dense();
   DevProblem dev() const;
   void upload_device();
   uint32_t row_stride() const { return (((N + 3) / 4) | 1u) * 4u; }
@@ -120,6 +126,7 @@ struct sqsg_job {
   bool on_device = false;
   uint32_t kmax = 0;      // largest species count over the problems
   bool lane_ok = false;   // the lane-per-candidate kernel applies
+  bool dense_ok = false;  // the dense MFMA contraction applies
   std::atomic<bool> cancel{false};
   // run state
   std::map<double, sqsg::LevelRecords> levels;  // levels at or below the keep threshold
@@ -180,5 +187,8 @@ void server_step(sqsg_server *s, int *active, int *queued);
 void server_poll(sqsg_server *s, uint64_t id, int *done, double *ts, double *ta, double *td);
 void server_destroy(sqsg_server *s);
 void ensure_device(sqsg_job *job);
+void ensure_dense(sqsg_job *job);
+// dense variant launch shape of a job: tile bound (kernel template) and 1-wave blocks
+uint32_t dense_tile_bound_of(const sqsg_job *job);
 
 }  // namespace sqsg

@@ -14,7 +14,7 @@
 // sequential, non-fused evaluation because results are keyed on exact objective values.
 #include <hip/hip_runtime.h>
 
-#include "device_common.hpp"
+#include "candidates.hpp"
 #include "kernels.hpp"
 
 namespace sqsg {
@@ -26,63 +26,6 @@ __device__ __forceinline__ void load_row(uint8_t *row, const DevProblem &pr) {
   uint32_t *dst = reinterpret_cast<uint32_t *>(row);
   const uint32_t words = (pr.N + 3) >> 2;
   for (uint32_t w = 0; w < words; ++w) dst[w] = src[w];
-}
-
-// Candidate `cand` of an MC stream: Fisher-Yates of the sorted configuration with draws
-// [cand * draws, (cand + 1) * draws) of the rapidrand stream (the stream is a counter:
-// state_k = seed + k * s0), windows in bounds order, i from w down to 2.
-__device__ __forceinline__ void shuffle_row(uint8_t *row, const DevProblem &pr, uint64_t cand) {
-  uint64_t st = pr.seed + (cand * static_cast<uint64_t>(pr.draws)) * kSecret0;
-  for (uint32_t b = 0; b < pr.nbounds; ++b) {
-    const uint32_t lo = pr.bounds[2 * b], hi = pr.bounds[2 * b + 1];
-    for (uint32_t i = hi - lo; i > 1; --i) {
-      const uint32_t p = static_cast<uint32_t>((static_cast<uint64_t>(draw32(st)) * i) >> 32);
-      const uint8_t t = row[lo + i - 1];
-      row[lo + i - 1] = row[lo + p];
-      row[lo + p] = t;
-    }
-  }
-}
-
-// permutation.h:52-78 on the window bounds[0..1], local species remapped (shuffle.h:69-88)
-__device__ void unrank_row(uint8_t *row, const DevProblem &pr, uint64_t rank) {
-  uint32_t hist[kMaxLocalSpecies];
-  for (uint32_t j = 0; j < kMaxLocalSpecies; ++j) hist[j] = pr.sys_hist[j];
-  const uint32_t lo = pr.bounds[0], w = pr.bounds[1] - pr.bounds[0];
-  uint64_t total = 1;
-  uint32_t placed = 0;
-  for (uint32_t j = 0; j < pr.sys_nspecies; ++j)
-    for (uint32_t k = 1; k <= hist[j]; ++k) total = muldiv(total, ++placed, k);
-  uint32_t k = 0;
-  for (uint32_t i = 0; i < w; ++i)
-    for (uint32_t j = 0; j < pr.sys_nspecies; ++j) {
-      const uint64_t sc = muldiv(total, hist[j], w - i);
-      if (rank <= sc) {
-        row[lo + k++] = pr.sys_rev[j];
-        total = sc;
-        --hist[j];
-        break;
-      }
-      rank -= sc;
-    }
-}
-
-// permutation.h:80-102 on the window
-__device__ void next_perm_row(uint8_t *row, uint32_t lo, uint32_t w) {
-  if (w < 2) return;
-  int k = static_cast<int>(w) - 2;
-  while (k >= 0 && row[lo + k] >= row[lo + k + 1]) --k;
-  if (k < 0) return;
-  int l = static_cast<int>(w) - 1;
-  while (row[lo + k] >= row[lo + l]) --l;
-  uint8_t t = row[lo + k];
-  row[lo + k] = row[lo + l];
-  row[lo + l] = t;
-  for (int a = k + 1, b = static_cast<int>(w) - 1; a < b; ++a, --b) {
-    t = row[lo + a];
-    row[lo + a] = row[lo + b];
-    row[lo + b] = t;
-  }
 }
 
 // count_bonds (optimization.h:74-77) over the pair list into the wave's LDS histogram
@@ -190,13 +133,14 @@ __global__ __launch_bounds__(256) void run_wave_kernel(RunParams p) {
             if (p.systematic) {
               if (r == 0) {
                 load_row(row, pr);
-                unrank_row(row, pr, cand + 1);
+                unrank_candidate(Strided<0>{row}, pr, cand + 1, 1u);
               } else {
-                next_perm_row(row, pr.bounds[0], pr.bounds[1] - pr.bounds[0]);
+                next_perm_candidate(Strided<0>{row}, pr.bounds[0], pr.bounds[1] - pr.bounds[0]);
               }
             } else {
               load_row(row, pr);
-              if (!(p.chunk_quirk && cand % p.chunk_size == 0)) shuffle_row(row, pr, cand);
+              if (!(p.chunk_quirk && cand % p.chunk_size == 0))
+                shuffle_candidate(Strided<0>{row}, pr, cand);
             }
           }
         }

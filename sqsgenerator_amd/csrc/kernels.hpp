@@ -28,6 +28,16 @@ hipError_t prepare_batch_lane(uint32_t kmax, size_t lds);
 // bits per packed neighbour-count field for a largest species count kmax
 uint32_t lane_field_bits(uint32_t kmax);
 
+// dense contraction on the MFMA cores; tile_bound = dense_tile_bound(32-site tiles)
+uint32_t dense_tile_bound(uint32_t tiles);
+uint32_t dense_groups(uint32_t tile_bound);  // 32-column tile groups per wavefront
+hipError_t launch_run_dense(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
+                            hipStream_t stream);
+hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32_t blocks,
+                              hipStream_t stream);
+hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds);
+hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStream_t stream);
+
 // order-preserving key of a double (unsigned order == numeric order)
 inline unsigned long long objective_key(double d) {
   unsigned long long b;

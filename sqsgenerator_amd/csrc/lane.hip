@@ -13,7 +13,7 @@
 // is complete, which is exactly the summation order of optimization.h:85-100.
 #include <hip/hip_runtime.h>
 
-#include "device_common.hpp"
+#include "candidates.hpp"
 #include "kernels.hpp"
 
 namespace sqsg {
@@ -38,63 +38,8 @@ template <int KM> struct LaneShape {
   static constexpr uint32_t ROW_BYTES_PER_UNIT = W * 256u / F;  // bytes per unit of a * F
 };
 
-// column access: element of site i of this lane's candidate
-__device__ __forceinline__ uint8_t &at(uint8_t *col, uint32_t i) { return col[i << 6]; }
-
 __device__ __forceinline__ void load_col(uint8_t *col, const DevProblem &pr) {
-  for (uint32_t i = 0; i < pr.N; ++i) at(col, i) = pr.species0F[i];
-}
-
-__device__ __forceinline__ void shuffle_col(uint8_t *col, const DevProblem &pr, uint64_t cand) {
-  uint64_t st = pr.seed + (cand * static_cast<uint64_t>(pr.draws)) * kSecret0;
-  for (uint32_t b = 0; b < pr.nbounds; ++b) {
-    const uint32_t lo = pr.bounds[2 * b], hi = pr.bounds[2 * b + 1];
-    for (uint32_t i = hi - lo; i > 1; --i) {
-      const uint32_t p = static_cast<uint32_t>((static_cast<uint64_t>(draw32(st)) * i) >> 32);
-      const uint8_t t = at(col, lo + i - 1);
-      at(col, lo + i - 1) = at(col, lo + p);
-      at(col, lo + p) = t;
-    }
-  }
-}
-
-__device__ void unrank_col(uint8_t *col, const DevProblem &pr, uint64_t rank) {
-  uint32_t hist[kMaxLocalSpecies];
-  for (uint32_t j = 0; j < kMaxLocalSpecies; ++j) hist[j] = pr.sys_hist[j];
-  const uint32_t lo = pr.bounds[0], w = pr.bounds[1] - pr.bounds[0];
-  uint64_t total = 1;
-  uint32_t placed = 0;
-  for (uint32_t j = 0; j < pr.sys_nspecies; ++j)
-    for (uint32_t k = 1; k <= hist[j]; ++k) total = muldiv(total, ++placed, k);
-  uint32_t k = 0;
-  for (uint32_t i = 0; i < w; ++i)
-    for (uint32_t j = 0; j < pr.sys_nspecies; ++j) {
-      const uint64_t sc = muldiv(total, hist[j], w - i);
-      if (rank <= sc) {
-        at(col, lo + k++) = static_cast<uint8_t>(pr.sys_rev[j] * pr.F);
-        total = sc;
-        --hist[j];
-        break;
-      }
-      rank -= sc;
-    }
-}
-
-__device__ void next_perm_col(uint8_t *col, uint32_t lo, uint32_t w) {
-  if (w < 2) return;
-  int k = static_cast<int>(w) - 2;
-  while (k >= 0 && at(col, lo + k) >= at(col, lo + k + 1)) --k;
-  if (k < 0) return;
-  int l = static_cast<int>(w) - 1;
-  while (at(col, lo + k) >= at(col, lo + l)) --l;
-  uint8_t t = at(col, lo + k);
-  at(col, lo + k) = at(col, lo + l);
-  at(col, lo + l) = t;
-  for (int a = k + 1, b = static_cast<int>(w) - 1; a < b; ++a, --b) {
-    t = at(col, lo + a);
-    at(col, lo + a) = at(col, lo + b);
-    at(col, lo + b) = t;
-  }
+  for (uint32_t i = 0; i < pr.N; ++i) col[i << 6] = pr.species0F[i];
 }
 
 template <int KM>
@@ -176,13 +121,6 @@ __device__ double score_lane(const uint8_t *col, const DevProblem &pr, uint32_t 
   return obj;
 }
 
-// Wave-local state of the tie-keeping filter: the L smallest distinct objectives this
-// wave has seen for the current job; its L-th bounds the job's global threshold.
-struct LevelFilter {
-  double lvl[kMaxLevels];
-  uint32_t n;
-};
-
 __device__ __forceinline__ void init_rows(const RunParams &p, uint8_t *wl, int lane) {
   // dummy site N of every problem: padding entries of the neighbour stream read it
   for (uint32_t q = 0; q < p.nprob; ++q)
@@ -209,13 +147,14 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
       if (p.systematic) {
         if (r == 0) {
           load_col(col, pr);
-          if (active) unrank_col(col, pr, cand + 1);
+          if (active) unrank_candidate(Strided<6>{col}, pr, cand + 1, pr.F);
         } else if (active) {
-          next_perm_col(col, pr.bounds[0], pr.bounds[1] - pr.bounds[0]);
+          next_perm_candidate(Strided<6>{col}, pr.bounds[0], pr.bounds[1] - pr.bounds[0]);
         }
       } else {
         load_col(col, pr);
-        if (active && !(p.chunk_quirk && cand % p.chunk_size == 0)) shuffle_col(col, pr, cand);
+        if (active && !(p.chunk_quirk && cand % p.chunk_size == 0))
+          shuffle_candidate(Strided<6>{col}, pr, cand);
       }
     }
     if (__ballot(active) == 0) break;  // the tail of a systematic segment
@@ -249,27 +188,7 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
         for (uint32_t q = 0; q < p.nprob; ++q) p.rec_sub[slot * p.nprob + q] = sub[q];
       }
     }
-    unsigned long long m = __ballot(pass);
-    bool changed = false;
-    while (m) {
-      const int l = __ffsll(m) - 1;
-      m &= m - 1;
-      const double o = __shfl(tot, l);
-      bool present = false;
-      for (uint32_t i = 0; i < lf.n; ++i) present |= (lf.lvl[i] == o);
-      if (present || !(lf.n < p.L || o < lf.lvl[lf.n - 1])) continue;
-      uint32_t pos = lf.n < p.L ? lf.n++ : lf.n - 1;
-      while (pos > 0 && lf.lvl[pos - 1] > o) {
-        lf.lvl[pos] = lf.lvl[pos - 1];
-        --pos;
-      }
-      lf.lvl[pos] = o;
-      changed = true;
-    }
-    if (changed && lf.n == p.L && lane == 0) {
-      const unsigned long long k = okey(lf.lvl[p.L - 1]);
-      if (k < g) atomicMin(p.thr_key, k);
-    }
+    offer_levels(lf, p, __ballot(pass), tot, g, lane);
   }
 }
 

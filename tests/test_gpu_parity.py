@@ -68,13 +68,16 @@ WORKLOADS = {
 }
 
 
-KERNELS = {"wave": 1, "lane": 2}
+KERNELS = {"wave": 1, "lane": 2, "dense": 3}
+SINGLE_PROBLEM = [n for n in WORKLOADS if not n.endswith("_split")]  # dense kernel domain
 
 
 @pytest.mark.parametrize("kernel", list(KERNELS))
 @pytest.mark.parametrize("name", list(WORKLOADS))
 def test_score_batch_bit_exact(gpu, name, kernel):
     print(f"[score_batch] {name} {kernel}", flush=True)
+    if kernel == "dense" and name not in SINGLE_PROBLEM:
+        pytest.skip("the dense kernel scores jobs of one sublattice problem")
     cfg = parse_config(WORKLOADS[name]())
     job = Job(cfg)
     rng = np.random.default_rng(20261203)
@@ -202,37 +205,38 @@ def _pack_signature(pack):
 
 
 @pytest.mark.parametrize("name", ["fcc32_3shells", "fcc256_quinary", "rocksalt216_split",
-                                  "hcp128_binary"])
+                                  "hcp128_binary", "fcc864_quinary"])
 def test_variants_agree_mc(gpu, name):
     """candidate t is the same configuration in every kernel (counter-based streams), so the
-    wave and lane kernels must return identical result packs for the same seed"""
+    wave, lane and dense kernels must return identical result packs for the same seed"""
     print(f"[variants] {name}", flush=True)
     d = WORKLOADS[name]()
-    d["iterations"] = 200_000
-    d["chunk_size"] = 50_000
+    d["iterations"] = 200_000 if name != "fcc864_quinary" else 20_000
+    d["chunk_size"] = min(d["iterations"], 50_000)
     packs = []
-    for kernel in (1, 2):
+    for kernel in (1, 2, 3) if name in SINGLE_PROBLEM else (1, 2):
         job = Job(parse_config(d))
         job.set_kernel(kernel)
         job.run()
         packs.append(_pack_signature(job.results()))
-        assert job.timing()[2] == 200_000
+        assert job.timing()[2] == d["iterations"]
         job.close()
-    assert packs[0] == packs[1]
+    for other in packs[1:]:
+        assert other == packs[0]
 
 
 def test_variants_agree_systematic(gpu):
     d = W.reference_test_16()
     del d["pair_weights"]
     packs = []
-    for kernel in (1, 2):
+    for kernel in (1, 2, 3):
         job = Job(parse_config(d))
         job.set_kernel(kernel)
         job.set_launch_size(1000)
         job.run()
         packs.append(_pack_signature(job.results()))
         job.close()
-    assert packs[0] == packs[1]
+    assert packs[0] == packs[1] == packs[2]
 
 
 def test_systematic_32_optimal_rank_set(gpu):
@@ -339,6 +343,54 @@ def test_server_results_alone_and_in_the_mix(gpu):
 def test_autotune_picks_a_variant(gpu):
     job = Job(parse_config(_small("fcc256_quinary")))
     kernel, rate = job.autotune()
-    assert kernel in (1, 2) and rate > 0
+    assert kernel in (1, 2, 3) and rate > 0
     job.run()
     assert job.timing()[2] == 131072
+
+
+@pytest.mark.parametrize("name", ["fcc256_quinary", "fcc864_quinary"])
+def test_large_cell_sparse_vs_dense(gpu, name):
+    """4096 uniformly random candidates scored by the LDS pair-list kernel and by the int8
+    one-hot x int8 shell-adjacency MFMA contraction: bit-exact bond counts and objectives,
+    and the oracle on a sample"""
+    print(f"[sparse-vs-dense] {name}", flush=True)
+    cfg = parse_config(WORKLOADS[name]())
+    job = Job(cfg)
+    ref = oracle_inputs(cfg, job)
+    rng = np.random.default_rng(4096)
+    cands = random_candidates(ref["packed"], ref["bounds"], 4096, rng)
+    b_lane, o_lane, s_lane = job.score(cands, 0, kernel=2)
+    b_dense, o_dense, s_dense = job.score(cands, 0, kernel=3)
+    assert np.array_equal(b_lane, b_dense)
+    assert np.array_equal(o_lane.view(np.uint64), o_dense.view(np.uint64))
+    assert np.array_equal(s_lane, s_dense)
+    for b in rng.choice(4096, 32, replace=False):
+        rb = O.count_bonds(ref["pairs"], cands[b], ref["S"], ref["K"])
+        assert np.array_equal(b_dense[b].astype(np.uint64), rb)
+    job.close()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_dense_matches_single_job_runs(gpu, use_graph):
+    """the persistent dense form (jobs of one tile bound share a launch) returns the packs of
+    the single-job dense runs, which equal the lane kernel's"""
+    print(f"[batch-dense] graph={use_graph}", flush=True)
+    names = ["fcc256_quinary", "fcc108_ternary", "bcc128_quaternary", "hcp128_binary",
+             "fcc32_3shells"]
+    alone = []
+    for name in names:
+        job = Job(parse_config(_small(name)))
+        job.set_kernel(2)
+        job.run()
+        alone.append(_pack_signature(job.results()))
+        job.close()
+    jobs = [Job(parse_config(_small(name))) for name in names]
+    for j in jobs:
+        j.set_kernel(3)
+        j.set_launch_size(32768)
+    rep = S.batch_run(jobs, use_graph=use_graph)
+    assert rep["candidates"] == 131072 * len(jobs)
+    for j, ref in zip(jobs, alone):
+        assert j.timing()[2] == 131072
+        assert _pack_signature(j.results()) == ref
+        j.close()
