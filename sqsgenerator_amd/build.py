@@ -20,9 +20,10 @@ ARCH = os.environ.get("SQSG_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-fvisibility=hidden", "-Wall", "-Wno-unused-function", "-I", str(SRC)]
-# device code: gfx950 only; packed FP32 instructions disabled
+# device code: gfx950 only; packed FP32 instructions disabled; fully unrolled tile loops of
+# the dense kernel exceed the default size limit of `#pragma unroll`
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+          "-munsafe-fp-atomics", "-mllvm", "-pragma-unroll-threshold=1000000"]
 
 
 def _hipcc() -> str:

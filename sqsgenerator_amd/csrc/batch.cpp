@@ -59,7 +59,7 @@ size_t job_footprint(const sqsg_job *job) {
   size_t b = 0;
   for (const auto &P : job->problems)
     b += P.pi.size() * 4 + P.lane_stream.size() * 4 + P.pref.size() * 24 + 2 * P.N + 1024 +
-         size_t(P.dtiles_n) * 1024 + P.dmask.size() * 4 + 32 * P.dT;
+         size_t(P.dtiles_n) * 1024 + 32 * P.dT;
   return b;
 }
 
@@ -250,8 +250,8 @@ class BatchEngine {
     const size_t block_lds = wave_lds;
     const bool dense = g.variant == SQSG_KERNEL_DENSE_MFMA;
     uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / std::max<size_t>(block_lds, 1));
-    // dense wavefronts fill a SIMD's register file: one per SIMD
-    per_cu = std::max(1u, std::min(per_cu, dense ? 4u : 16u));
+    per_cu = std::max(1u, std::min(per_cu, 16u));
+    if (dense) per_cu = dense_waves_per_cu(g.key, block_lds, true);
     const uint32_t blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
     BatchParams bp{};
     bp.jobs = g.d_jobs.as<const RunParams>();

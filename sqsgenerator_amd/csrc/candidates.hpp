@@ -22,9 +22,10 @@ template <uint32_t SH> struct Strided {
   __device__ __forceinline__ uint8_t &operator[](uint32_t i) const { return p[i << SH]; }
 };
 
-template <uint32_t SH>
-__device__ __forceinline__ void shuffle_candidate(Strided<SH> v, const DevProblem &pr,
-                                                  uint64_t cand) {
+// PR is DevProblem, possibly qualified with the constant address space (a job table entry
+// read with scalar loads)
+template <uint32_t SH, class PR>
+__device__ __forceinline__ void shuffle_candidate(Strided<SH> v, const PR &pr, uint64_t cand) {
   uint64_t st = pr.seed + (cand * static_cast<uint64_t>(pr.draws)) * kSecret0;
   for (uint32_t b = 0; b < pr.nbounds; ++b) {
     const uint32_t lo = pr.bounds[2 * b], hi = pr.bounds[2 * b + 1];
@@ -38,9 +39,8 @@ __device__ __forceinline__ void shuffle_candidate(Strided<SH> v, const DevProble
 }
 
 // writes the packed species of rank `rank` (1-based) times `scale` into the window
-template <uint32_t SH>
-__device__ void unrank_candidate(Strided<SH> v, const DevProblem &pr, uint64_t rank,
-                                 uint32_t scale) {
+template <uint32_t SH, class PR>
+__device__ void unrank_candidate(Strided<SH> v, const PR &pr, uint64_t rank, uint32_t scale) {
   uint32_t hist[kMaxLocalSpecies];
   for (uint32_t j = 0; j < kMaxLocalSpecies; ++j) hist[j] = pr.sys_hist[j];
   const uint32_t lo = pr.bounds[0], w = pr.bounds[1] - pr.bounds[0];
@@ -90,7 +90,8 @@ struct LevelFilter {
 // Offers the objectives of the lanes in `pass` (a ballot) to the filter and lowers the
 // job's threshold when the wave holds L levels below it.  `g` is the threshold read
 // before the candidates were tested.
-__device__ __forceinline__ void offer_levels(LevelFilter &lf, const RunParams &p,
+template <class RP>
+__device__ __forceinline__ void offer_levels(LevelFilter &lf, const RP &p,
                                              unsigned long long pass, double tot,
                                              unsigned long long g, int lane) {
   bool changed = false;

@@ -19,6 +19,13 @@
 //    the 16 registers pack straight into GEMM2's B operand (Y <= 127 is checked on the host).
 // Shells are processed in order and each shell's objective terms are summed as soon as its
 // counts are out, which keeps the FP64 summation order of optimization.h:85-100.
+//
+// The MFMA part of a shell is straight-line code over the TT x TT upper tile grid (TT is
+// the kernel's tile bound, T <= TT the problem's tile count): every tile load is issued a
+// fixed number of grid positions ahead into a register ring and every MFMA runs; tiles
+// past the problem's T x T grid read zeros (bounds-checked buffer loads) and zero tiles of
+// the grid are stored as zeros, so they add nothing.  Without branches the scheduler can
+// overlap the row-end packing (accumulator reads, byte packing) with the next row's MFMAs.
 #include <hip/hip_runtime.h>
 
 #include "candidates.hpp"
@@ -51,31 +58,76 @@ __device__ __forceinline__ unsigned long long low_mask(uint32_t n) {
   return n >= 64 ? ~0ull : ((1ull << n) - 1ull);
 }
 
+// read-only problem data through the constant address space: uniform loads become scalar
+// loads, which do not wait behind the tile stream's vector loads
+template <class T> __device__ __forceinline__ T ldc(const T *p, uint32_t i) {
+  return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T &constant_ref(const T *p,
+                                                                                   uint32_t i) {
+  return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
+// Tile-stream prefetch depth (in grid positions) of a tile bound; TT (TT + 1) / 2 grid
+// positions per shell must be a multiple of it so every shell reuses the same ring slots.
+template <int TT> struct DenseRing {
+  static constexpr int P = TT * (TT + 1) / 2;
+  static constexpr int D = TT >= 32 ? 16 : TT >= 28 ? 14 : TT >= 16 ? 8 : TT >= 8 ? 6
+                         : TT >= 4 ? 5 : P;
+  static_assert(P % D == 0, "ring depth must divide the grid positions of a shell");
+};
+
+// launch shapes: tile bound TT and 32-column tile groups G per wavefront (X holds G * TT
+// one-hot tiles in registers)
+template <int TT_, int G_> struct DenseShape {
+  static constexpr int TT = TT_, G = G_;
+};
+template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
+  switch (tile_bound) {
+    case 1: return f(DenseShape<1, 4>{});
+    case 2: return f(DenseShape<2, 4>{});
+    case 4: return f(DenseShape<4, 4>{});
+    case 8: return f(DenseShape<8, 4>{});
+    case 16: return f(DenseShape<16, 2>{});
+    case 28: return f(DenseShape<28, 1>{});
+    case 32: return f(DenseShape<32, 1>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// byte offset of upper tile (I, J) of shell s in the stream (every I <= J < T tile is
+// stored, zero tiles included), or an offset past the end (the load returns zeros)
+__device__ __forceinline__ uint32_t tile_offset(uint32_t s, uint32_t I, uint32_t J, uint32_t S,
+                                                uint32_t T) {
+  if (s >= S || I >= T || J >= T) return 0x7fff0000u;
+  const uint32_t per_shell = T * (T + 1) / 2;
+  return (s * per_shell + I * T - I * (I - 1) / 2 + (J - I)) * 1024u;
+}
+
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
-// bytes, padding sites 0xff).  Passes whose bit in `live` is clear are skipped.  Returns
-// the objective of the lane's own candidate (lane < npass * CW) and, when given, writes
-// its ordered bonds and SRO cube.
-template <int TT, int G>
-__device__ __forceinline__ double dense_score(const DevProblem &pr, const uint8_t *rows, uint32_t stride,
-                              uint32_t *cnt, uint32_t CW, uint32_t npass,
-                              unsigned long long live, int lane, int32_t *bonds, double *sro) {
-  const uint32_t K = pr.K, KK = K * K, T = pr.dT;
+// bytes, padding sites 0xff); `dump` is G * 1024 words of LDS.  Passes whose bit in `live`
+// is clear are skipped.  Returns the objective of the lane's own candidate
+// (lane < npass * CW) and, when given, writes its ordered bonds and SRO cube.
+template <int TT, int G, class PR>
+__device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
+                                              uint32_t stride, uint32_t *dump, uint32_t CW,
+                                              uint32_t npass, unsigned long long live, int lane,
+                                              int32_t *bonds, double *sro) {
+  constexpr int D = DenseRing<TT>::D, P = DenseRing<TT>::P;
+  const uint32_t K = pr.K, KK = K * K, T = pr.dT, S = pr.S;
   const uint32_t g = CW / G;
   const uint32_t h = static_cast<uint32_t>(lane) >> 5, n = static_cast<uint32_t>(lane) & 31u;
   const uint32_t cl = n / K, b = n - cl * K;
   const bool col_ok = cl < g;
   const uint32_t bb = b * 0x01010101u;
-  uint32_t keep = 0;  // accumulator registers of GEMM2 on this lane's diagonal block
-#pragma unroll
-  for (uint32_t r = 0; r < 16; ++r) {
-    const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
-    if (col_ok && m / K == cl) keep |= 1u << r;
-  }
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<uint32_t *>(pr.dtiles), static_cast<short>(0),
       static_cast<int>(pr.dtiles_n * 1024u), 0x00020000);
   const uint32_t toff = static_cast<uint32_t>(lane) * 16u;
   const uint32_t my_pass = static_cast<uint32_t>(lane) / CW;
+  const uint32_t my_cc = static_cast<uint32_t>(lane) - my_pass * CW;  // candidate in its pass
+  const uint32_t my_group = my_cc / g, my_c = my_cc - my_group * g;
   double obj = 0.0;
   for (uint32_t pass = 0; pass < npass; ++pass) {
     if (!((live >> pass) & 1ull)) continue;
@@ -97,63 +149,85 @@ __device__ __forceinline__ double dense_score(const DevProblem &pr, const uint8_
         X[gi][J] = x;
       }
     }
-    uint32_t t = 0;  // position in the tile stream
-    v4i cur = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, toff, 0, 0));
-    v4i nxt = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, toff + 1024u, 0, 0));
-    for (uint32_t s = 0; s < pr.S; ++s) {
-      v16i C[G];
+    // ring slot q % D holds grid position q (row-major over I <= J < TT) of the shell.
+    // Grid positions are stepped incrementally so that every index folds to a constant
+    // once the loops are unrolled.
+    v4i ring[D];
+    {
+      int I2 = 0, J2 = 0;
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        ring[q] = __builtin_bit_cast(
+            v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, tile_offset(0, I2, J2, S, T) + toff,
+                                                       0, 0));
+        if (++J2 == TT) ++I2, J2 = I2;
+      }
+    }
+    for (uint32_t s = 0; s < S; ++s) {
+      v16i C[G], Y[G];
 #pragma unroll
       for (int gi = 0; gi < G; ++gi) C[gi] = v16i{};
+      int I = 0, J = 0;             // current grid position
+      int I2 = 0, J2 = 0, s2 = 0;   // the position D ahead (s2 = 1: next shell)
 #pragma unroll
-      for (int I = 0; I < TT; ++I) {
-        if (static_cast<uint32_t>(I) >= T) continue;
-        const uint32_t mask = pr.dmask[s * T + I];  // bits J < T only
-        v16i Y[G];
+      for (int k = 0; k < D; ++k)
+        if (++J2 == TT) {
+          ++I2, J2 = I2;
+          if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
+        }
 #pragma unroll
-        for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
+      for (int q = 0; q < P; ++q) {
+        if (J == I) {
 #pragma unroll
-        for (int J = I; J < TT; ++J) {
-          if ((mask >> J) & 1u) {
-            const v4i u = cur;
-            cur = nxt;
-            nxt = __builtin_bit_cast(
-                v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, (t + 2) * 1024u + toff, 0, 0));
-            ++t;
-#pragma unroll
-            for (int gi = 0; gi < G; ++gi)
-              Y[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(u, X[gi][J], Y[gi], 0, 0, 0);
-          }
+          for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
+        }
+        const v4i u = ring[q % D];
+        ring[q % D] = __builtin_bit_cast(
+            v4i, __builtin_amdgcn_raw_buffer_load_b128(
+                     rs, tile_offset(s + s2, I2, J2, S, T) + toff, 0, 0));
+        if (++J2 == TT) {
+          ++I2, J2 = I2;
+          if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
         }
 #pragma unroll
         for (int gi = 0; gi < G; ++gi)
-          C[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(X[gi][I], pack_counts(Y[gi]), C[gi],
-                                                        0, 0, 0);
+          Y[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(u, X[gi][J], Y[gi], 0, 0, 0);
+        if (J == TT - 1) {  // row I complete: C += X_I^T Y_I
+#pragma unroll
+          for (int gi = 0; gi < G; ++gi)
+            C[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(X[gi][I], pack_counts(Y[gi]), C[gi],
+                                                          0, 0, 0);
+        }
+        if (++J == TT) ++I, J = I;
       }
-      // diagonal blocks -> counts [candidate of the pass][a][b]
+      // accumulators -> LDS [group][lane][register]; the diagonal blocks are read below
 #pragma unroll
       for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-        for (uint32_t r = 0; r < 16; ++r)
-          if ((keep >> r) & 1u) {
-            const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
-            cnt[gi * g * KK + K * m + b] = static_cast<uint32_t>(C[gi][r]);
-          }
+        for (int w = 0; w < 4; ++w)
+          *reinterpret_cast<uint4 *>(dump + (gi * 64 + lane) * 16 + 4 * w) =
+              make_uint4(C[gi][4 * w], C[gi][4 * w + 1], C[gi][4 * w + 2], C[gi][4 * w + 3]);
       wave_sync();
       if (my_pass == pass) {
-        const uint32_t *cc = cnt + (static_cast<uint32_t>(lane) - pass * CW) * KK;
-        const double *pref = pr.pref + s * KK;
-        const double *pw = pr.pw + s * KK;
-        const double *tgt = pr.tgt + s * KK;
+        // count (a, b) of candidate c of group gi: accumulator row m = cK + a, column
+        // n = cK + b, held by lane 32 ((m / 4) % 2) + n in register 4 (m / 8) + m % 4
+        const uint32_t *dg = dump + my_group * 1024u;
+        auto count = [&](uint32_t a, uint32_t b2) {
+          const uint32_t m = my_c * K + a, nn = my_c * K + b2;
+          return dg[(32 * ((m >> 2) & 1u) + nn) * 16 + 4 * (m >> 3) + (m & 3u)];
+        };
+        const uint32_t o = s * KK;
         for (uint32_t xi = 0; xi < K; ++xi) {
-          const uint32_t cxx = cc[xi * K + xi];
-          const double sxx = 1.0 - static_cast<double>(cxx) * pref[xi * K + xi];
-          obj += pw[xi * K + xi] * absolute(sxx - tgt[xi * K + xi]);
+          const uint32_t cxx = count(xi, xi);
+          const double sxx = 1.0 - static_cast<double>(cxx) * ldc(pr.pref, o + xi * K + xi);
+          obj += ldc(pr.pw, o + xi * K + xi) * absolute(sxx - ldc(pr.tgt, o + xi * K + xi));
           if (sro) sro[(s * K + xi) * K + xi] = sxx;
           if (bonds) bonds[(s * K + xi) * K + xi] = static_cast<int32_t>(cxx);
           for (uint32_t eta = xi + 1; eta < K; ++eta) {
-            const uint32_t cab = cc[xi * K + eta], cba = cc[eta * K + xi];
-            const double v = 1.0 - static_cast<double>(cab + cba) * pref[xi * K + eta];
-            obj += pw[xi * K + eta] * absolute(v - tgt[xi * K + eta]);
+            const uint32_t cab = count(xi, eta), cba = count(eta, xi);
+            const uint32_t ix = o + xi * K + eta;
+            const double v = 1.0 - static_cast<double>(cab + cba) * ldc(pr.pref, ix);
+            obj += ldc(pr.pw, ix) * absolute(v - ldc(pr.tgt, ix));
             if (sro) sro[(s * K + xi) * K + eta] = v, sro[(s * K + eta) * K + xi] = v;
             if (bonds) {
               bonds[(s * K + xi) * K + eta] = static_cast<int32_t>(cab);
@@ -168,7 +242,7 @@ __device__ __forceinline__ double dense_score(const DevProblem &pr, const uint8_
   return obj;
 }
 
-__device__ __forceinline__ void load_dense_row(uint8_t *row, const DevProblem &pr) {
+template <class PR> __device__ __forceinline__ void load_dense_row(uint8_t *row, const PR &pr) {
   const uint32_t *src = reinterpret_cast<const uint32_t *>(pr.species0D);
   uint32_t *dst = reinterpret_cast<uint32_t *>(row);
   for (uint32_t w = 0; w < 8 * pr.dT; ++w) dst[w] = src[w];
@@ -177,14 +251,15 @@ __device__ __forceinline__ void load_dense_row(uint8_t *row, const DevProblem &p
 // One segment of a job: gen_group lanes x p.rounds candidates starting at sbeg (the
 // candidate numbering of the lane kernel with 64 replaced by gen_group); candidates at or
 // beyond `end` are inactive.
-template <int TT, int G>
-__device__ __forceinline__ void dense_segment(const RunParams &p, uint64_t sbeg, uint64_t end, uint8_t *wl,
-                              int lane, LevelFilter &lf, unsigned long long &scored) {
-  const DevProblem &pr = p.prob[0];
+template <int TT, int G, class RP>
+__device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64_t end,
+                                              uint8_t *wl, int lane, LevelFilter &lf,
+                                              unsigned long long &scored) {
+  const auto &pr = p.prob[0];
   const uint32_t GEN = p.gen_group, CW = p.dense_cw, stride = p.dense_stride;
   const uint32_t npass = GEN / CW;
   uint8_t *rows = wl + p.off_rows[0];
-  uint32_t *cnt = reinterpret_cast<uint32_t *>(wl + p.off_hist);
+  uint32_t *dump = reinterpret_cast<uint32_t *>(wl + p.off_hist);
   const bool gl = static_cast<uint32_t>(lane) < GEN;
   uint8_t *row = rows + (gl ? static_cast<uint32_t>(lane) : 0u) * stride;
   for (uint32_t r = 0; r < p.rounds; ++r) {
@@ -212,7 +287,7 @@ __device__ __forceinline__ void dense_segment(const RunParams &p, uint64_t sbeg,
       if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
     wave_sync();
     const double tot =
-        dense_score<TT, G>(pr, rows, stride, cnt, CW, npass, live, lane, nullptr, nullptr);
+        dense_score<TT, G>(pr, rows, stride, dump, CW, npass, live, lane, nullptr, nullptr);
     scored += static_cast<unsigned long long>(__popcll(am));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -263,18 +338,22 @@ __global__ __launch_bounds__(64) void batch_dense_kernel(BatchParams bp) {
     if (lane == 0) item = atomicAdd(bp.next_item, 1u);
     item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
     if (item >= num_items) break;  // every wave leaves once the list is drained
-    const WorkItem wi = bp.items[item];
-    const RunParams &p = bp.jobs[wi.job];
-    if (wi.job != last_job) {
+    // the job table is read-only during the launch: scalar loads through the constant
+    // address space keep every job parameter uniform
+    const auto &wi = constant_ref(bp.items, item);
+    const uint32_t job = wi.job;
+    const uint64_t wbeg = wi.begin, wend = wi.end;
+    const auto &p = constant_ref(bp.jobs, job);
+    if (job != last_job) {
       if (last_job != 0xffffffffu && lane == 0 && scored)
         atomicAdd(bp.jobs[last_job].scored, scored);
       scored = 0;
       lf.n = 0;
-      last_job = wi.job;
+      last_job = job;
     }
     const uint64_t per_seg = static_cast<uint64_t>(p.gen_group) * p.rounds;
-    for (uint64_t sbeg = wi.begin; sbeg < wi.end; sbeg += per_seg)
-      dense_segment<TT, G>(p, sbeg, wi.end, lds, lane, lf, scored);
+    for (uint64_t sbeg = wbeg; sbeg < wend; sbeg += per_seg)
+      dense_segment<TT, G>(p, sbeg, wend, lds, lane, lf, scored);
   }
   if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
 }
@@ -286,7 +365,7 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   const DevProblem &pr = p.prob;
   const uint32_t GEN = p.dense_gen, CW = p.dense_cw, npass = GEN / CW;
   uint8_t *rows = lds + p.off_row;
-  uint32_t *cnt = reinterpret_cast<uint32_t *>(lds + p.off_hist);
+  uint32_t *dump = reinterpret_cast<uint32_t *>(lds + p.off_hist);
   const uint32_t nb = pr.S * pr.K * pr.K;
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * GEN + lane;
   const bool gl = static_cast<uint32_t>(lane) < GEN;
@@ -301,7 +380,7 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   for (uint32_t q = 0; q < npass; ++q)
     if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
   wave_sync();
-  const double obj = dense_score<TT, G>(pr, rows, p.dense_stride, cnt, CW, npass, live, lane,
+  const double obj = dense_score<TT, G>(pr, rows, p.dense_stride, dump, CW, npass, live, lane,
                                         active && p.bonds ? p.bonds + b * nb : nullptr,
                                         active && p.sro ? p.sro + b * nb : nullptr);
   if (active) p.objective[b] = obj;
@@ -339,46 +418,56 @@ template <int TT, int G> hipError_t score_dense_k(const ScoreParams &p, hipStrea
 
 }  // namespace
 
-uint32_t dense_tile_bound(uint32_t tiles) { return tiles <= 8 ? 8 : tiles <= 16 ? 16 : 32; }
-uint32_t dense_groups(uint32_t tile_bound) { return tile_bound == 8 ? 4 : tile_bound == 16 ? 2 : 1; }
+uint32_t dense_tile_bound(uint32_t tiles) {
+  for (uint32_t tt : {1u, 2u, 4u, 8u, 16u, 28u, 32u})
+    if (tiles <= tt) return tt;
+  return 0;
+}
+uint32_t dense_groups(uint32_t tile_bound) { return tile_bound <= 8 ? 4 : tile_bound <= 16 ? 2 : 1; }
+
+uint32_t dense_waves_per_cu(uint32_t tile_bound, size_t wave_lds, bool persistent) {
+  int n = 0;
+  const hipError_t e = with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    const void *fn = persistent
+                         ? reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G>)
+                         : reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G>);
+    hipError_t r = dense_lds(fn, wave_lds);
+    if (r != hipSuccess) return r;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, wave_lds);
+  });
+  return e == hipSuccess && n > 0 ? static_cast<uint32_t>(n) : 1u;
+}
 
 hipError_t launch_run_dense(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
                             hipStream_t s) {
-  switch (tile_bound) {
-    case 8: return run_dense_k<8, 4>(p, blocks, s);
-    case 16: return run_dense_k<16, 2>(p, blocks, s);
-    case 32: return run_dense_k<32, 1>(p, blocks, s);
-    default: return hipErrorInvalidValue;
-  }
+  return with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    return run_dense_k<Sh::TT, Sh::G>(p, blocks, s);
+  });
 }
 
 hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds) {
-  switch (tile_bound) {
-    case 8: return dense_lds(reinterpret_cast<const void *>(&batch_dense_kernel<8, 4>), lds);
-    case 16: return dense_lds(reinterpret_cast<const void *>(&batch_dense_kernel<16, 2>), lds);
-    case 32: return dense_lds(reinterpret_cast<const void *>(&batch_dense_kernel<32, 1>), lds);
-    default: return hipErrorInvalidValue;
-  }
+  return with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    return dense_lds(reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G>), lds);
+  });
 }
 
 hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32_t blocks,
                               hipStream_t s) {
-  switch (tile_bound) {
-    case 8: return batch_dense_k<8, 4>(bp, blocks, s);
-    case 16: return batch_dense_k<16, 2>(bp, blocks, s);
-    case 32: return batch_dense_k<32, 1>(bp, blocks, s);
-    default: return hipErrorInvalidValue;
-  }
+  return with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    return batch_dense_k<Sh::TT, Sh::G>(bp, blocks, s);
+  });
 }
 
 hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStream_t s) {
   if (p.B == 0) return hipSuccess;
-  switch (tile_bound) {
-    case 8: return score_dense_k<8, 4>(p, s);
-    case 16: return score_dense_k<16, 2>(p, s);
-    case 32: return score_dense_k<32, 1>(p, s);
-    default: return hipErrorInvalidValue;
-  }
+  return with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    return score_dense_k<Sh::TT, Sh::G>(p, s);
+  });
 }
 
 }  // namespace sqsg

@@ -21,10 +21,9 @@ struct DevProblem {
   const double *tgt;          // [S][K][K]
   const uint32_t *terms;      // T objective terms in reference order: s | xi << 8 | eta << 16
   // dense contraction: 32 x 32 int8 tiles of the upper shell adjacency in the A-operand
-  // layout of v_mfma_i32_32x32x32_i8, streamed in (shell, row tile, column tile) order;
-  // dmask[s * dT + I] has bit J set when tile (s, I, J) is in the stream
+  // layout of v_mfma_i32_32x32x32_i8, every tile I <= J < dT of every shell, in (shell,
+  // row tile, column tile) order
   const uint32_t *dtiles;     // 256 words per tile
-  const uint32_t *dmask;
   const uint8_t *species0D;   // sorted configuration, 32 * dT bytes, padding sites 0xff
   uint32_t dT;                // 32-site tiles
   uint32_t dtiles_n;
@@ -61,7 +60,7 @@ struct RunParams {
   // LDS layout of one wave (bytes)
   uint32_t wave_lds;
   uint32_t off_rows[kMaxProblems];
-  uint32_t off_hist, off_terms, off_obj;  // dense: off_hist holds one shell's counts
+  uint32_t off_hist, off_terms, off_obj;  // dense: off_hist holds the accumulator dump
   // tie-keeping threshold and record sink
   unsigned long long *thr_key;
   unsigned int *rec_count;

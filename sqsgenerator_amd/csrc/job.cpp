@@ -212,15 +212,16 @@ void Problem::build_lane(uint32_t field_bits) {
 
 // Tiles in the A-operand layout of v_mfma_i32_32x32x32_i8: lane l, byte q of tile (s, I, J)
 // is U_s[32I + perm(l % 32)][32J + 16(l / 32) + q] with perm(m) = 16((m / 4) % 2) +
-// 4(m / 8) + m % 4 (the accumulator row order of the 32 x 32 result).  Tiles without a
-// pair are left out of the stream and their bit in dmask is clear.
+// 4(m / 8) + m % 4 (the accumulator row order of the 32 x 32 result).  Every tile with
+// I <= J < T is stored (shell, then I, then J), zero tiles included, so a tile's position
+// is a function of (s, I, J) alone.
 void Problem::build_dense() {
   if (d_dtiles.size()) return;
   dT = (N + 31) / 32;
   const uint32_t n32 = 32 * dT;
   std::vector<uint8_t> sh(size_t(n32) * n32, 0xff);
   for (size_t q = 0; q < pi.size(); ++q) sh[size_t(pi[q]) * n32 + pj[q]] = static_cast<uint8_t>(ps[q]);
-  dmask.assign(size_t(S) * dT, 0u);
+  dtiles_nonzero = 0;
   std::vector<uint8_t> tiles;
   for (uint32_t s = 0; s < S; ++s)
     for (uint32_t I = 0; I < dT; ++I)
@@ -232,10 +233,10 @@ void Problem::build_dense() {
               any = true;
               break;
             }
-        if (!any) continue;
-        dmask[size_t(s) * dT + I] |= 1u << J;
         const size_t base = tiles.size();
         tiles.resize(base + 1024, 0);
+        if (!any) continue;
+        ++dtiles_nonzero;
         for (uint32_t l = 0; l < 64; ++l) {
           const uint32_t m = l % 32;
           const uint32_t i = 32 * I + 16 * ((m / 4) % 2) + 4 * (m / 8) + m % 4;
@@ -249,7 +250,6 @@ void Problem::build_dense() {
   std::vector<uint8_t> sp(n32, 0xff);
   std::copy(species_packed.begin(), species_packed.end(), sp.begin());
   upload(d_dtiles, tiles, 16);
-  upload(d_dmask, dmask);
   upload(d_species0D, sp, 16);
 }
 
@@ -307,7 +307,6 @@ DevProblem Problem::dev() const {
   d.species0F = d_species0F.as<const uint8_t>();
   d.F = F;
   d.dtiles = d_dtiles.as<const uint32_t>();
-  d.dmask = d_dmask.as<const uint32_t>();
   d.species0D = d_species0D.as<const uint8_t>();
   d.dT = dT;
   d.dtiles_n = dtiles_n;
@@ -504,22 +503,22 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   uint32_t off = 0;
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
-    // rows of gen_group candidates (npass passes of dense_cw) + one shell's counts
+    // rows of gen_group candidates (npass passes of dense_cw) + the accumulator dump
     const Problem &P = job->problems[0];
     const uint32_t T = (P.N + 31) / 32, G = dense_groups(dense_tile_bound(T));
     const uint32_t g = std::min(32 / P.K, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
-    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4);
+    const uint32_t cnt_bytes = G * 4096;  // accumulator dump [G][64 lanes][16]
     uint32_t npass = 64 / p.dense_cw;
     while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 40 * 1024) --npass;
     p.gen_group = npass * p.dense_cw;
     p.off_rows[0] = 0;
     p.off_hist = align(p.gen_group * p.dense_stride);
     p.wave_lds = p.off_hist + cnt_bytes;
-    // one wavefront per block; the register file holds one dense wavefront per SIMD
-    const uint32_t per_cu = std::max(1u, std::min(4u, static_cast<uint32_t>((160 * 1024) / p.wave_lds)));
-    *blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
+    // one wavefront per block, as many as registers and LDS keep resident
+    *blocks = static_cast<uint32_t>(device_cu_count()) *
+              dense_waves_per_cu(dense_tile_bound(T), p.wave_lds, false);
     return p;
   }
   if (variant == SQSG_KERNEL_SPARSE_LANE) {

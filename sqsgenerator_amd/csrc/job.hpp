@@ -83,8 +83,8 @@ struct Problem {
   uint32_t dT = 0;          // 32-site tiles
   uint32_t max_upper = 0;   // largest count of upper neighbours of one site in one shell
   uint32_t dtiles_n = 0;
-  std::vector<uint32_t> dmask;
-  DevBuf d_dtiles, d_dmask, d_species0D;
+  uint32_t dtiles_nonzero = 0;  // tiles holding at least one pair
+  DevBuf d_dtiles, d_species0D;
   void build_dense();
   DevProblem dev() const;
   void upload_device();

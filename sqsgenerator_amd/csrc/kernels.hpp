@@ -31,6 +31,8 @@ uint32_t lane_field_bits(uint32_t kmax);
 // dense contraction on the MFMA cores; tile_bound = dense_tile_bound(32-site tiles)
 uint32_t dense_tile_bound(uint32_t tiles);
 uint32_t dense_groups(uint32_t tile_bound);  // 32-column tile groups per wavefront
+// resident 1-wave blocks per CU of the dense kernel (registers and LDS), at least 1
+uint32_t dense_waves_per_cu(uint32_t tile_bound, size_t wave_lds, bool persistent);
 hipError_t launch_run_dense(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
                             hipStream_t stream);
 hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32_t blocks,
