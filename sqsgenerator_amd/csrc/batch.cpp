@@ -2,10 +2,10 @@
 //
 // BatchEngine keeps a set of resident jobs (their inputs, thresholds and record sinks stay
 // in device memory) and advances all of them with one persistent launch per step: each
-// step gives every active job its next slice of candidates, split into work items that the
-// persistent wavefronts pull from a shared counter.  Jobs are grouped by kernel variant and
-// its specialisation (largest species count for the lane kernel, tile bound for the dense
-// MFMA kernel); each group has fixed device tables
+// step gives every active job its next slice of candidates as one work item, and the
+// persistent wavefronts claim the items one segment at a time through per-item counters.
+// Jobs are grouped by kernel variant and its specialisation (largest species count for the
+// lane kernel, tile bound for the dense MFMA kernel); each group has fixed device tables
 // and a captured graph {counter reset kernel, persistent kernel} that is replayed every
 // step.  Between steps the host drains the record sinks, tightens each job's threshold and
 // admits or retires jobs, which is what the serving loop builds on.
@@ -25,7 +25,7 @@ namespace sqsg {
 namespace {
 
 constexpr uint32_t kGroupJobs = 256;      // resident jobs per species-count group
-constexpr uint32_t kGroupItems = 1u << 17;
+constexpr uint32_t kGroupItems = kGroupJobs;  // one work item per resident job and step
 
 struct Group;
 
@@ -37,7 +37,7 @@ struct Slot {
   DevBuf thr, count, scored;
   uint64_t next = 0;         // first candidate of the next slice
   uint64_t slice = 0;        // candidates per step
-  uint64_t item = 0;         // candidates per work item
+  uint64_t chunk = 0;        // candidates a wavefront claims at once (one segment)
   uint64_t issued_end = 0;   // end of the slice issued in the current step
   size_t footprint = 0;      // device bytes held
   bool done = false;
@@ -46,7 +46,7 @@ struct Slot {
 struct Group {
   int variant = 0;
   uint32_t key = 0;  // lane: largest species count; dense: tile bound
-  DevBuf d_jobs, d_items, d_counts, d_next;
+  DevBuf d_jobs, d_items, d_counts, d_claims;
   hipGraphExec_t exec = nullptr;
   uint32_t graph_wave_lds = 0, graph_blocks = 0;
   std::vector<Slot *> members;
@@ -102,9 +102,8 @@ class BatchEngine {
     } else {
       s->p.rounds = 1;
     }
-    const uint64_t seg = gen * s->p.rounds;
-    s->item = std::max<uint64_t>(seg, ((uint64_t(1) << 22) / std::max<uint64_t>(work, 1)) / seg * seg);
-    s->slice = std::max(s->slice, s->item);
+    s->chunk = gen * s->p.rounds;
+    s->slice = std::max(s->slice, s->chunk);
     s->p.rec_stride = job->stride;
     s->sink.ensure(static_cast<uint32_t>(std::min<uint64_t>(s->slice, 1u << 14)), job->stride,
                    s->p.nprob);
@@ -215,7 +214,7 @@ class BatchEngine {
       g->d_jobs.alloc(kGroupJobs * sizeof(RunParams));
       g->d_items.alloc(kGroupItems * sizeof(WorkItem));
       g->d_counts.alloc(2 * sizeof(uint32_t));
-      g->d_next.alloc(sizeof(unsigned int));
+      g->d_claims.alloc(kGroupItems * sizeof(unsigned int));
     }
     return *g;
   }
@@ -231,10 +230,8 @@ class BatchEngine {
       wave_lds = std::max(wave_lds, s->p.wave_lds);
       const uint64_t end = std::min(s->job->iterations, s->next + s->slice);
       s->issued_end = s->job->cancel.load() ? s->next : end;
-      for (uint64_t b = s->next; b < s->issued_end; b += s->item) {
-        if (items.size() == kGroupItems) throw std::out_of_range("work item table full");
-        items.push_back(WorkItem{k, 0, b, std::min(s->issued_end, b + s->item)});
-      }
+      if (s->issued_end > s->next)
+        items.push_back(WorkItem{k, static_cast<uint32_t>(s->chunk), s->next, s->issued_end});
     }
     last_items = static_cast<uint32_t>(items.size());
     const uint32_t n = last_items;
@@ -259,7 +256,8 @@ class BatchEngine {
     bp.num_items = g.d_counts.as<const uint32_t>();
     bp.num_jobs = g.d_counts.as<const uint32_t>() + 1;
     bp.wave_lds = wave_lds;
-    bp.next_item = g.d_next.as<unsigned int>();
+    bp.claims = g.d_claims.as<unsigned int>();
+    bp.max_items = kGroupItems;
     SQSG_HIP(dense ? prepare_batch_dense(g.key, block_lds) : prepare_batch_lane(g.key, block_lds));
     auto launch = [&]() {
       return dense ? launch_batch_dense(bp, g.key, blocks, stream_)

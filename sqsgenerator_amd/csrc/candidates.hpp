@@ -116,5 +116,28 @@ __device__ __forceinline__ void offer_levels(LevelFilter &lf, const RP &p,
   }
 }
 
+// Next chunk of a persistent launch's work list for this wavefront: walks the items in
+// order and claims one chunk of the first item that has any left.  Returns false when the
+// list is drained; `it` carries the wavefront's position between calls.
+__device__ __forceinline__ bool claim_chunk(const BatchParams &bp, uint32_t num_items,
+                                            uint32_t &it, int lane, uint32_t &job,
+                                            uint64_t &cbeg, uint64_t &cend) {
+  while (it < num_items) {
+    uint32_t c = 0;
+    if (lane == 0) c = atomicAdd(bp.claims + it, 1u);
+    c = __builtin_amdgcn_readfirstlane(__shfl(c, 0));
+    const WorkItem wi = bp.items[it];
+    const uint64_t b = wi.begin + static_cast<uint64_t>(c) * wi.chunk;
+    if (b < wi.end) {
+      job = __builtin_amdgcn_readfirstlane(wi.job);
+      cbeg = b;
+      cend = b + wi.chunk < wi.end ? b + wi.chunk : wi.end;
+      return true;
+    }
+    ++it;
+  }
+  return false;
+}
+
 }  // namespace
 }  // namespace sqsg

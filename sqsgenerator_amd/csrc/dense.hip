@@ -78,20 +78,23 @@ template <int TT> struct DenseRing {
   static_assert(P % D == 0, "ring depth must divide the grid positions of a shell");
 };
 
-// launch shapes: tile bound TT and 32-column tile groups G per wavefront (X holds G * TT
-// one-hot tiles in registers)
-template <int TT_, int G_> struct DenseShape {
-  static constexpr int TT = TT_, G = G_;
+// launch shapes: tile bound TT, 32-column tile groups G per wavefront (X holds G * TT
+// one-hot tiles in registers) and resident wavefronts W per SIMD the registers are
+// budgeted for.  With two per SIMD (256 registers each), one wavefront generates candidates
+// or sums objectives while the other keeps the matrix core busy; large tile bounds need the
+// whole register file for the one-hot tiles and the prefetch ring.
+template <int TT_, int G_, int W_> struct DenseShape {
+  static constexpr int TT = TT_, G = G_, W = W_;
 };
 template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
   switch (tile_bound) {
-    case 1: return f(DenseShape<1, 4>{});
-    case 2: return f(DenseShape<2, 4>{});
-    case 4: return f(DenseShape<4, 4>{});
-    case 8: return f(DenseShape<8, 4>{});
-    case 16: return f(DenseShape<16, 2>{});
-    case 28: return f(DenseShape<28, 1>{});
-    case 32: return f(DenseShape<32, 1>{});
+    case 1: return f(DenseShape<1, 4, 2>{});
+    case 2: return f(DenseShape<2, 4, 2>{});
+    case 4: return f(DenseShape<4, 2, 2>{});
+    case 8: return f(DenseShape<8, 3, 2>{});
+    case 16: return f(DenseShape<16, 2, 1>{});
+    case 28: return f(DenseShape<28, 1, 1>{});
+    case 32: return f(DenseShape<32, 1, 1>{});
     default: return hipErrorInvalidValue;
   }
 }
@@ -307,8 +310,8 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
   }
 }
 
-template <int TT, int G>
-__global__ __launch_bounds__(64) void run_dense_kernel(RunParams p) {
+template <int TT, int G, int W>
+__global__ __launch_bounds__(64, W) void run_dense_kernel(RunParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
   const uint64_t per_seg = static_cast<uint64_t>(p.gen_group) * p.rounds;
@@ -317,32 +320,33 @@ __global__ __launch_bounds__(64) void run_dense_kernel(RunParams p) {
   LevelFilter lf;
   lf.n = 0;
   unsigned long long scored = 0;
-  for (uint64_t seg = blockIdx.x; seg < nseg; seg += gridDim.x)
+  // segments are pulled from a counter: wavefronts that start late (more blocks than fit
+  // at once) or run slow do not hold up the launch
+  for (;;) {
+    uint32_t seg = 0;
+    if (lane == 0) seg = atomicAdd(p.next_seg, 1u);
+    seg = __builtin_amdgcn_readfirstlane(__shfl(seg, 0));
+    if (seg >= nseg) break;
     dense_segment<TT, G>(p, p.cand_begin + seg * per_seg, p.cand_end, lds, lane, lf, scored);
+  }
   if (lane == 0 && scored) atomicAdd(p.scored, scored);
 }
 
-// Persistent form: every wavefront pulls work items (job, candidate range) from a shared
-// counter until the list is exhausted; the jobs of a launch share the tile bound TT.
-template <int TT, int G>
-__global__ __launch_bounds__(64) void batch_dense_kernel(BatchParams bp) {
+// Persistent form: every wavefront claims one-segment chunks of the jobs' candidate ranges
+// until the work list is drained; the jobs of a launch share the tile bound TT.
+template <int TT, int G, int W>
+__global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
   const uint32_t num_items = *bp.num_items;
-  uint32_t last_job = 0xffffffffu;
+  uint32_t last_job = 0xffffffffu, it = 0, job = 0;
+  uint64_t cbeg = 0, cend = 0;
   LevelFilter lf;
   lf.n = 0;
   unsigned long long scored = 0;
-  for (;;) {
-    uint32_t item = 0;
-    if (lane == 0) item = atomicAdd(bp.next_item, 1u);
-    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
-    if (item >= num_items) break;  // every wave leaves once the list is drained
+  while (claim_chunk(bp, num_items, it, lane, job, cbeg, cend)) {
     // the job table is read-only during the launch: scalar loads through the constant
     // address space keep every job parameter uniform
-    const auto &wi = constant_ref(bp.items, item);
-    const uint32_t job = wi.job;
-    const uint64_t wbeg = wi.begin, wend = wi.end;
     const auto &p = constant_ref(bp.jobs, job);
     if (job != last_job) {
       if (last_job != 0xffffffffu && lane == 0 && scored)
@@ -351,9 +355,7 @@ __global__ __launch_bounds__(64) void batch_dense_kernel(BatchParams bp) {
       lf.n = 0;
       last_job = job;
     }
-    const uint64_t per_seg = static_cast<uint64_t>(p.gen_group) * p.rounds;
-    for (uint64_t sbeg = wbeg; sbeg < wend; sbeg += per_seg)
-      dense_segment<TT, G>(p, sbeg, wend, lds, lane, lf, scored);
+    dense_segment<TT, G>(p, cbeg, cend, lds, lane, lf, scored);
   }
   if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
 }
@@ -392,18 +394,19 @@ hipError_t dense_lds(const void *fn, size_t bytes) {
                              static_cast<int>(bytes));
 }
 
-template <int TT, int G> hipError_t run_dense_k(const RunParams &p, uint32_t blocks,
-                                                hipStream_t s) {
-  const hipError_t e = dense_lds(reinterpret_cast<const void *>(&run_dense_kernel<TT, G>),
+template <int TT, int G, int W> hipError_t run_dense_k(const RunParams &p, uint32_t blocks,
+                                                       hipStream_t s) {
+  const hipError_t e = dense_lds(reinterpret_cast<const void *>(&run_dense_kernel<TT, G, W>),
                                  p.wave_lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((run_dense_kernel<TT, G>), dim3(blocks), dim3(64), p.wave_lds, s, p);
+  hipLaunchKernelGGL((run_dense_kernel<TT, G, W>), dim3(blocks), dim3(64), p.wave_lds, s, p);
   return hipGetLastError();
 }
 
-template <int TT, int G> hipError_t batch_dense_k(const BatchParams &bp, uint32_t blocks,
-                                                  hipStream_t s) {
-  hipLaunchKernelGGL((batch_dense_kernel<TT, G>), dim3(blocks), dim3(64), bp.wave_lds, s, bp);
+template <int TT, int G, int W> hipError_t batch_dense_k(const BatchParams &bp, uint32_t blocks,
+                                                         hipStream_t s) {
+  hipLaunchKernelGGL((batch_dense_kernel<TT, G, W>), dim3(blocks), dim3(64), bp.wave_lds, s,
+                     bp);
   return hipGetLastError();
 }
 
@@ -423,15 +426,22 @@ uint32_t dense_tile_bound(uint32_t tiles) {
     if (tiles <= tt) return tt;
   return 0;
 }
-uint32_t dense_groups(uint32_t tile_bound) { return tile_bound <= 8 ? 4 : tile_bound <= 16 ? 2 : 1; }
+uint32_t dense_groups(uint32_t tile_bound) {
+  uint32_t g = 0;
+  (void)with_dense_shape(tile_bound, [&](auto sh) {
+    g = decltype(sh)::G;
+    return hipSuccess;
+  });
+  return g;
+}
 
 uint32_t dense_waves_per_cu(uint32_t tile_bound, size_t wave_lds, bool persistent) {
   int n = 0;
   const hipError_t e = with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
     const void *fn = persistent
-                         ? reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G>)
-                         : reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G>);
+                         ? reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W>)
+                         : reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G, Sh::W>);
     hipError_t r = dense_lds(fn, wave_lds);
     if (r != hipSuccess) return r;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, wave_lds);
@@ -443,14 +453,14 @@ hipError_t launch_run_dense(const RunParams &p, uint32_t tile_bound, uint32_t bl
                             hipStream_t s) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return run_dense_k<Sh::TT, Sh::G>(p, blocks, s);
+    return run_dense_k<Sh::TT, Sh::G, Sh::W>(p, blocks, s);
   });
 }
 
 hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return dense_lds(reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G>), lds);
+    return dense_lds(reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W>), lds);
   });
 }
 
@@ -458,7 +468,7 @@ hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32
                               hipStream_t s) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return batch_dense_k<Sh::TT, Sh::G>(bp, blocks, s);
+    return batch_dense_k<Sh::TT, Sh::G, Sh::W>(bp, blocks, s);
   });
 }
 

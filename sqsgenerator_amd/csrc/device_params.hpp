@@ -71,12 +71,14 @@ struct RunParams {
   uint32_t rec_cap;
   uint32_t rec_stride;
   unsigned long long *scored;
+  unsigned int *next_seg;      // dense single-job kernel: segment counter (zero at launch)
 };
 
-// A slice of one job's candidate range handed to a persistent wavefront.
+// One job's candidate range of a persistent step; wavefronts claim it `chunk` candidates
+// (one segment) at a time through the item's claim counter.
 struct WorkItem {
   uint32_t job;
-  uint32_t pad;
+  uint32_t chunk;
   uint64_t begin, end;
 };
 
@@ -86,7 +88,8 @@ struct BatchParams {
   const uint32_t *num_items;   // device scalars {items, jobs}: a captured graph stays valid
   const uint32_t *num_jobs;    // while the tables change between replays
   uint32_t wave_lds;           // LDS of one wavefront: the largest job's layout
-  unsigned int *next_item;     // work-item counter
+  unsigned int *claims;        // per-item chunk counters (zeroed by the reset kernel)
+  uint32_t max_items;          // rows of the item table
 };
 
 struct ScoreParams {

@@ -511,7 +511,8 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     p.dense_stride = 32 * T + 16;
     const uint32_t cnt_bytes = G * 4096;  // accumulator dump [G][64 lanes][16]
     uint32_t npass = 64 / p.dense_cw;
-    while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 40 * 1024) --npass;
+    // at most 20 KiB per wavefront: eight resident wavefronts per CU (two per SIMD)
+    while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 20 * 1024) --npass;
     p.gen_group = npass * p.dense_cw;
     p.off_rows[0] = 0;
     p.off_hist = align(p.gen_group * p.dense_stride);
@@ -640,12 +641,13 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   RecordSink sink;
   sink.ensure(static_cast<uint32_t>(std::min<uint64_t>(launch, 1u << 16)), stride, p.nprob);
   DevBuf d_thr(sizeof(unsigned long long)), d_count(sizeof(unsigned int)),
-      d_scored(sizeof(unsigned long long));
+      d_scored(sizeof(unsigned long long)), d_next(sizeof(unsigned int));
   unsigned long long thr = objective_key(std::numeric_limits<double>::infinity());
   SQSG_HIP(hipMemcpy(d_thr.as<void>(), &thr, sizeof(thr), hipMemcpyHostToDevice));
   p.thr_key = d_thr.as<unsigned long long>();
   p.rec_count = d_count.as<unsigned int>();
   p.scored = d_scored.as<unsigned long long>();
+  p.next_seg = d_next.as<unsigned int>();
   const auto t_loop = clock::now();
   job->stats.timing_chunk_setup_ns =
       std::chrono::duration_cast<std::chrono::nanoseconds>(t_loop - t_start).count();
@@ -658,6 +660,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     sink.bind(p);
     SQSG_HIP(hipMemsetAsync(d_count.as<void>(), 0, sizeof(unsigned int), job->stream));
     SQSG_HIP(hipMemsetAsync(d_scored.as<void>(), 0, sizeof(unsigned long long), job->stream));
+    SQSG_HIP(hipMemsetAsync(d_next.as<void>(), 0, sizeof(unsigned int), job->stream));
     SQSG_HIP(hipEventRecord(job->ev0, job->stream));
     SQSG_HIP(launch_variant(p, job, variant, blocks, W, job->stream));
     SQSG_HIP(hipEventRecord(job->ev1, job->stream));

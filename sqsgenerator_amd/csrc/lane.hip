@@ -213,9 +213,9 @@ __global__ __launch_bounds__(256) void run_lane_kernel(RunParams p) {
   if (lane == 0 && scored) atomicAdd(p.scored, scored);
 }
 
-// Persistent form: every wavefront pulls work items (job, candidate range) from a shared
-// counter until the list is exhausted; jobs of any size, species and shell count (up to
-// KM species) share the launch.  Each job has its own threshold and record sink.
+// Persistent form: every wavefront claims one-segment chunks of the jobs' candidate ranges
+// until the work list is drained; jobs of any size, species and shell count (up to KM
+// species) share the launch.  Each job has its own threshold and record sink.
 template <int KM>
 __global__ __launch_bounds__(256) void batch_lane_kernel(BatchParams bp) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -223,38 +223,32 @@ __global__ __launch_bounds__(256) void batch_lane_kernel(BatchParams bp) {
   const uint32_t wave = threadIdx.x >> 6;
   uint8_t *wl = lds + wave * bp.wave_lds;
   const uint32_t num_items = *bp.num_items;
-  uint32_t last_job = 0xffffffffu;
+  uint32_t last_job = 0xffffffffu, it = 0, job = 0;
+  uint64_t cbeg = 0, cend = 0;
   LevelFilter lf;
   lf.n = 0;
   unsigned long long scored = 0;
-  for (;;) {
-    uint32_t item = 0;
-    if (lane == 0) item = atomicAdd(bp.next_item, 1u);
-    item = __shfl(item, 0);
-    if (item >= num_items) break;  // every wave leaves once the list is drained
-    const WorkItem wi = bp.items[item];
-    const RunParams &p = bp.jobs[wi.job];
-    if (wi.job != last_job) {
+  while (claim_chunk(bp, num_items, it, lane, job, cbeg, cend)) {
+    const RunParams &p = bp.jobs[job];
+    if (job != last_job) {
       if (last_job != 0xffffffffu && lane == 0 && scored)
         atomicAdd(bp.jobs[last_job].scored, scored);
       scored = 0;
       lf.n = 0;
-      last_job = wi.job;
+      last_job = job;
       init_rows(p, wl, lane);
     }
     uint32_t *hl = reinterpret_cast<uint32_t *>(wl + p.off_hist) + lane;
-    const uint64_t per_seg = 64ull * p.rounds;
-    for (uint64_t sbeg = wi.begin; sbeg < wi.end; sbeg += per_seg)
-      lane_segment<KM>(p, sbeg, wi.end, wl, hl, lane, lf, scored);
+    lane_segment<KM>(p, cbeg, cend, wl, hl, lane, lf, scored);
   }
   if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
 }
 
-// zeroes the item counter and every job's record and scored counters (a kernel, so the
-// reset can sit inside a captured graph)
+// zeroes the item claim counters and every job's record and scored counters (a kernel, so
+// the reset can sit inside a captured graph)
 __global__ void batch_reset_kernel(BatchParams bp, uint32_t max_jobs) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) *bp.next_item = 0u;
+  if (t < bp.max_items && t < *bp.num_items) bp.claims[t] = 0u;
   if (t < max_jobs && t < *bp.num_jobs) {
     *bp.jobs[t].rec_count = 0u;
     *bp.jobs[t].scored = 0ull;
@@ -324,7 +318,8 @@ template <int KM> hipError_t launch_batch_lane_k(const BatchParams &bp, uint32_t
 }  // namespace
 
 hipError_t launch_batch_reset(const BatchParams &bp, uint32_t max_jobs, hipStream_t s) {
-  const uint32_t threads = 256, blocks = (max_jobs + threads) / threads;
+  const uint32_t n = max_jobs > bp.max_items ? max_jobs : bp.max_items;
+  const uint32_t threads = 256, blocks = (n + threads - 1) / threads;
   hipLaunchKernelGGL(batch_reset_kernel, dim3(blocks), dim3(threads), 0, s, bp, max_jobs);
   return hipGetLastError();
 }
