@@ -7,7 +7,8 @@ configuration -> shell x species^2 bond counts -> FP64 objective -> tie-keeping 
 
   * plain: the single-job wavefront-per-candidate kernel, one launch per step;
   * batched (the reported value): the persistent, graph-captured multi-job form running
-    four such jobs (different seeds) together; one step = one persistent launch;
+    four such jobs (different seeds) together with the kernel the job picks by default
+    (the exact int8 MFMA contraction for this cell); one step = one persistent launch;
   * serving: Poisson arrivals of mixed SQS jobs on the continuous-batching server, p50/p99
     job latency at the stated arrival rate;
   * cpu_baseline: the oracle's restatement of the reference worker loop on the host cores.
@@ -29,6 +30,26 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 METRIC = "candidates scored/sec (256-site 5-species FCC, 7 shells); job p99 latency"
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
+TRAFFIC_FILE = ROOT / "profiles" / "batch_dense_kernel_fcc256_traffic.json"
+
+
+def dense_ops_per_candidate(N, K, S):
+    """int8 ops of the dense one-hot contraction of one candidate, S (2 N^2 K + 2 N K^2)"""
+    return S * (2 * N * N * K + 2 * N * K * K)
+
+
+def measured_traffic(candidates_per_launch):
+    """HBM bytes per launch from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), if it matches this
+    launch size"""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if t.get("candidates_per_launch") != candidates_per_launch:
+        return None
+    return t.get("hbm_bytes_per_launch")
 
 
 def cpu_model() -> str:
@@ -205,6 +226,8 @@ def main() -> int:
         jobs = [make(steps * per_step // nb, seed=7 + k, launch=per_step // nb)
                 for k in range(nb)]
         for j in jobs:
+            j.set_kernel(L.KERNEL_AUTO)
+        for j in jobs:
             j.score(j.problem(0)["species_packed"][None, :])  # inputs resident in HBM
         return jobs
 
@@ -227,14 +250,16 @@ def main() -> int:
     value = scored / wall
     avg_launch_s = rep["kernel_ms"] / 1e3 / rep["launches"]
     P, N = info["num_pairs"], info["num_sites"]
-    bytes_per_cand = 6 * P + N + 8
-    achieved = bytes_per_cand * (scored / rep["launches"]) / avg_launch_s / 1e9
+    K, S = info["num_species"], info["num_shells"]
+    per_launch = scored // rep["launches"]
+    ops_per_cand = dense_ops_per_candidate(N, K, S)
+    achieved = ops_per_cand * per_launch / avg_launch_s / 1e12
     kernel_rate = scored / (rep["kernel_ms"] / 1e3)
     out = {
         "metric": METRIC, "value": value, "unit": "candidates/s", "n_gpus": 1,
         "steps": rep["launches"], "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / rep["launches"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32+f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "i8 MFMA->i32 counts, f64 objective",
         "data": "synthetic (random MC candidates of the generated cell)",
         "config": {"workload": "fcc4x4x4-256site-quinary-7shell-mc", "sites": N,
                    "species": info["num_species"], "shells": info["num_shells"], "pairs": P,
@@ -244,12 +269,15 @@ def main() -> int:
                    "plain_kernel": plain,
                    "speedup_vs_plain_kernel":
                        kernel_rate / plain["kernel_candidates_per_s"] if plain else None},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "batch_lane_kernel<5>", "avg_launch_ms": avg_launch_s * 1e3,
-                     "algorithmic_bytes_per_candidate": bytes_per_cand,
-                     "note": "the pair list is reused on chip, so the kernel is LDS/VALU "
-                             "bound; rocprof counters are in profiles/"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
+                     "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS,
+                     "traffic": measured_traffic(per_launch),
+                     "kernel": "batch_dense_kernel<8,3,2>", "avg_launch_ms": avg_launch_s * 1e3,
+                     "algorithmic_ops_per_candidate": ops_per_cand,
+                     "candidates_per_launch": per_launch,
+                     "note": "int8 ops (2 per multiply-accumulate) of the dense one-hot "
+                             "contraction S(2N^2K + 2NK^2) per candidate; launch time from HIP "
+                             "events around the graph launch on the engine's stream"},
     }
     if not args.no_serve:
         out["serving"] = serving(args.serve_rate, args.serve_seconds, 20261203,

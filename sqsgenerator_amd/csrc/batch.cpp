@@ -39,6 +39,8 @@ struct Slot {
   uint64_t slice = 0;        // candidates per step
   uint64_t chunk = 0;        // candidates a wavefront claims at once (one segment)
   uint64_t issued_end = 0;   // end of the slice issued in the current step
+  unsigned int h_count = 0;  // host copies of the device counters after a step
+  unsigned long long h_scored = 0, h_thr = 0;
   size_t footprint = 0;      // device bytes held
   bool done = false;
 };
@@ -48,6 +50,11 @@ struct Group {
   uint32_t key = 0;  // lane: largest species count; dense: tile bound
   DevBuf d_jobs, d_items, d_counts, d_claims;
   hipGraphExec_t exec = nullptr;
+  uint32_t blocks = 0;     // of the current step's launch
+  BatchParams bp{};
+  std::vector<RunParams> h_table;  // host sources of the step's uploads
+  std::vector<WorkItem> h_items;
+  uint32_t h_counts[2] = {0, 0};
   uint32_t graph_wave_lds = 0, graph_blocks = 0;
   std::vector<Slot *> members;
   ~Group() {
@@ -134,30 +141,37 @@ class BatchEngine {
     std::vector<sqsg_job *> finished;
     if (slots_.empty()) return finished;
     const auto t0 = std::chrono::steady_clock::now();
+    // tables first, then the launches bracketed by the step's events
+    std::vector<Group *> live;
+    for (auto &[k, g] : groups_)
+      if (!g->members.empty()) live.push_back(prepare_group(*g));
     SQSG_HIP(hipEventRecord(ev0_, stream_));
-    for (auto &[k, g] : groups_) {
-      if (g->members.empty()) continue;
-      launch_group(*g);
-    }
+    for (Group *g : live) launch_group(*g);
     SQSG_HIP(hipEventRecord(ev1_, stream_));
     SQSG_HIP(hipStreamSynchronize(stream_));
     float ms = 0;
     SQSG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
     kernel_ms += ms;
     ++launches;
+    // every slot's counters and threshold in one round trip
+    for (auto &s : slots_) {
+      if (s->issued_end == s->next) continue;
+      SQSG_HIP(hipMemcpyAsync(&s->h_count, s->count.as<void>(), sizeof(s->h_count),
+                              hipMemcpyDeviceToHost, stream_));
+      SQSG_HIP(hipMemcpyAsync(&s->h_scored, s->scored.as<void>(), sizeof(s->h_scored),
+                              hipMemcpyDeviceToHost, stream_));
+      SQSG_HIP(hipMemcpyAsync(&s->h_thr, s->thr.as<void>(), sizeof(s->h_thr),
+                              hipMemcpyDeviceToHost, stream_));
+    }
+    SQSG_HIP(hipStreamSynchronize(stream_));
     for (auto &s : slots_) {
       sqsg_job *job = s->job;
       if (s->issued_end == s->next) {
         if (s->next >= job->iterations || job->cancel.load()) s->done = true;
         continue;
       }
-      unsigned int count = 0;
-      unsigned long long scored = 0;
-      SQSG_HIP(hipMemcpyAsync(&count, s->count.as<void>(), sizeof(count), hipMemcpyDeviceToHost,
-                              stream_));
-      SQSG_HIP(hipMemcpyAsync(&scored, s->scored.as<void>(), sizeof(scored),
-                              hipMemcpyDeviceToHost, stream_));
-      SQSG_HIP(hipStreamSynchronize(stream_));
+      const unsigned int count = s->h_count;
+      const unsigned long long scored = s->h_scored;
       job->launches++;
       job->kernel_ms += ms;  // shared launch: every resident job saw the whole step
       if (count > s->sink.cap) {  // widen and issue the same slice again next step
@@ -170,10 +184,14 @@ class BatchEngine {
       job->scored += scored;
       candidates += scored;
       s->sink.drain(job, count, s->p.nprob, stream_);
+      // keep the device threshold at the tighter of its own bound and the exact host one
+      // (ordered before the next launch on the same stream; h_thr outlives the copy)
       const unsigned long long hk = objective_key(job->vkeep);
-      unsigned long long dk = 0;
-      SQSG_HIP(hipMemcpy(&dk, s->thr.as<void>(), sizeof(dk), hipMemcpyDeviceToHost));
-      if (hk < dk) SQSG_HIP(hipMemcpy(s->thr.as<void>(), &hk, sizeof(hk), hipMemcpyHostToDevice));
+      if (hk < s->h_thr) {
+        s->h_thr = hk;
+        SQSG_HIP(hipMemcpyAsync(s->thr.as<void>(), &s->h_thr, sizeof(s->h_thr),
+                                hipMemcpyHostToDevice, stream_));
+      }
       s->next = s->issued_end;
       job->stats.finished = job->scored;
       if (s->next >= job->iterations || job->cancel.load()) s->done = true;
@@ -219,9 +237,12 @@ class BatchEngine {
     return *g;
   }
 
-  void launch_group(Group &g) {
-    std::vector<RunParams> table(g.members.size());
-    std::vector<WorkItem> items;
+  // uploads the group's job table and work items for this step
+  Group *prepare_group(Group &g) {
+    std::vector<RunParams> &table = g.h_table;
+    std::vector<WorkItem> &items = g.h_items;
+    table.resize(g.members.size());
+    items.clear();
     uint32_t wave_lds = 0;
     for (uint32_t k = 0; k < g.members.size(); ++k) {
       Slot *s = g.members[k];
@@ -240,17 +261,19 @@ class BatchEngine {
     if (n)
       SQSG_HIP(hipMemcpyAsync(g.d_items.as<void>(), items.data(), n * sizeof(WorkItem),
                               hipMemcpyHostToDevice, stream_));
-    const uint32_t counts[2] = {n, static_cast<uint32_t>(g.members.size())};
-    SQSG_HIP(hipMemcpyAsync(g.d_counts.as<void>(), counts, sizeof(counts), hipMemcpyHostToDevice,
-                            stream_));
-    SQSG_HIP(hipStreamSynchronize(stream_));  // `table`/`items` are stack buffers
+    g.h_counts[0] = n;
+    g.h_counts[1] = static_cast<uint32_t>(g.members.size());
+    // the host tables stay untouched until the step's end synchronisation
+    SQSG_HIP(hipMemcpyAsync(g.d_counts.as<void>(), g.h_counts, sizeof(g.h_counts),
+                            hipMemcpyHostToDevice, stream_));
     const size_t block_lds = wave_lds;
     const bool dense = g.variant == SQSG_KERNEL_DENSE_MFMA;
     uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / std::max<size_t>(block_lds, 1));
     per_cu = std::max(1u, std::min(per_cu, 16u));
     if (dense) per_cu = dense_waves_per_cu(g.key, block_lds, true);
     const uint32_t blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
-    BatchParams bp{};
+    BatchParams &bp = g.bp;
+    bp = BatchParams{};
     bp.jobs = g.d_jobs.as<const RunParams>();
     bp.items = g.d_items.as<const WorkItem>();
     bp.num_items = g.d_counts.as<const uint32_t>();
@@ -258,24 +281,16 @@ class BatchEngine {
     bp.wave_lds = wave_lds;
     bp.claims = g.d_claims.as<unsigned int>();
     bp.max_items = kGroupItems;
+    g.blocks = blocks;
     SQSG_HIP(dense ? prepare_batch_dense(g.key, block_lds) : prepare_batch_lane(g.key, block_lds));
-    auto launch = [&]() {
-      return dense ? launch_batch_dense(bp, g.key, blocks, stream_)
-                   : launch_batch_lane(bp, g.key, blocks, 1, stream_);
-    };
-    if (!use_graph_) {
-      SQSG_HIP(launch_batch_reset(bp, kGroupJobs, stream_));
-      SQSG_HIP(launch());
-      return;
-    }
-    if (!g.exec || g.graph_wave_lds != wave_lds || g.graph_blocks != blocks) {
+    if (use_graph_ && (!g.exec || g.graph_wave_lds != wave_lds || g.graph_blocks != blocks)) {
       // the graph covers every table row (num_jobs is a kernel argument of the reset)
       if (g.exec) SQSG_HIP(hipGraphExecDestroy(g.exec));
       g.exec = nullptr;
       hipGraph_t graph = nullptr;
       SQSG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
       const hipError_t e1 = launch_batch_reset(bp, kGroupJobs, stream_);
-      const hipError_t e2 = launch();
+      const hipError_t e2 = launch_kernel(g);
       SQSG_HIP(hipStreamEndCapture(stream_, &graph));
       SQSG_HIP(e1);
       SQSG_HIP(e2);
@@ -284,6 +299,22 @@ class BatchEngine {
       SQSG_HIP(e3);
       g.graph_wave_lds = wave_lds;
       g.graph_blocks = blocks;
+    }
+    return &g;
+  }
+
+  hipError_t launch_kernel(const Group &g) {
+    return g.variant == SQSG_KERNEL_DENSE_MFMA
+               ? launch_batch_dense(g.bp, g.key, g.blocks, stream_)
+               : launch_batch_lane(g.bp, g.key, g.blocks, 1, stream_);
+  }
+
+  // the step's launches only (tables, attributes and graphs are prepared beforehand)
+  void launch_group(Group &g) {
+    if (!use_graph_) {
+      SQSG_HIP(launch_batch_reset(g.bp, kGroupJobs, stream_));
+      SQSG_HIP(launch_kernel(g));
+      return;
     }
     SQSG_HIP(hipGraphLaunch(g.exec, stream_));
   }

@@ -294,7 +294,12 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
     scored += static_cast<unsigned long long>(__popcll(am));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const bool pass = active && okey(tot) <= g;
+    // the wave's own L-th distinct level bounds the final levels as well: a candidate above
+    // it has L better distinct objectives and cannot be kept, so it is not recorded
+    const bool below = active && okey(tot) <= g;
+    offer_levels(lf, p, __ballot(below), tot, g, lane);
+    const unsigned long long lk = lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
+    const bool pass = below && okey(tot) <= lk;
     if (pass) {
       const uint32_t slot = atomicAdd(p.rec_count, 1u);
       if (slot < p.rec_cap) {
@@ -305,7 +310,6 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
         p.rec_sub[slot] = tot;
       }
     }
-    offer_levels(lf, p, __ballot(pass), tot, g, lane);
     wave_sync();  // rows are rewritten by the next round
   }
 }

@@ -465,8 +465,10 @@ sqsg_job *create_job(const sqsg_config &c) {
 
 int resolve_kernel(const sqsg_job *job, int requested) {
   switch (requested) {
-    case SQSG_KERNEL_AUTO:
-      return job->lane_ok ? SQSG_KERNEL_SPARSE_LANE : SQSG_KERNEL_SPARSE_WAVE;
+    case SQSG_KERNEL_AUTO:  // exact everywhere; fastest first where it applies
+      return job->dense_ok  ? SQSG_KERNEL_DENSE_MFMA
+             : job->lane_ok ? SQSG_KERNEL_SPARSE_LANE
+                            : SQSG_KERNEL_SPARSE_WAVE;
     case SQSG_KERNEL_SPARSE_WAVE:
       return requested;
     case SQSG_KERNEL_SPARSE_LANE:
@@ -665,10 +667,12 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     SQSG_HIP(launch_variant(p, job, variant, blocks, W, job->stream));
     SQSG_HIP(hipEventRecord(job->ev1, job->stream));
     unsigned int count = 0;
-    unsigned long long scored = 0;
+    unsigned long long scored = 0, dk = 0;
     SQSG_HIP(hipMemcpyAsync(&count, d_count.as<void>(), sizeof(count), hipMemcpyDeviceToHost,
                             job->stream));
     SQSG_HIP(hipMemcpyAsync(&scored, d_scored.as<void>(), sizeof(scored), hipMemcpyDeviceToHost,
+                            job->stream));
+    SQSG_HIP(hipMemcpyAsync(&dk, d_thr.as<void>(), sizeof(dk), hipMemcpyDeviceToHost,
                             job->stream));
     SQSG_HIP(hipStreamSynchronize(job->stream));
     float ms = 0;
@@ -684,8 +688,6 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     sink.drain(job, count, p.nprob, job->stream);
     // keep the device threshold at the tighter of its own bound and the exact host one
     const unsigned long long hk = objective_key(job->vkeep);
-    unsigned long long dk = 0;
-    SQSG_HIP(hipMemcpy(&dk, d_thr.as<void>(), sizeof(dk), hipMemcpyDeviceToHost));
     if (hk < dk) SQSG_HIP(hipMemcpy(d_thr.as<void>(), &hk, sizeof(hk), hipMemcpyHostToDevice));
     job->stats.finished = job->scored;
     begin = end;

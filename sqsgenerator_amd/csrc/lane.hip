@@ -171,7 +171,12 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
     scored += static_cast<unsigned long long>(__popcll(__ballot(active)));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const bool pass = active && okey(tot) <= g;
+    // the wave's own L-th distinct level bounds the final levels as well: a candidate above
+    // it has L better distinct objectives and cannot be kept, so it is not recorded
+    const bool below = active && okey(tot) <= g;
+    offer_levels(lf, p, __ballot(below), tot, g, lane);
+    const unsigned long long lk = lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
+    const bool pass = below && okey(tot) <= lk;
     if (pass) {
       const uint32_t slot = atomicAdd(p.rec_count, 1u);
       if (slot < p.rec_cap) {
@@ -188,7 +193,6 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
         for (uint32_t q = 0; q < p.nprob; ++q) p.rec_sub[slot * p.nprob + q] = sub[q];
       }
     }
-    offer_levels(lf, p, __ballot(pass), tot, g, lane);
   }
 }
 
