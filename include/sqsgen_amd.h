@@ -52,7 +52,11 @@ extern "C" {
 #define SQSG_SUBLATTICE_INTERACT 0
 #define SQSG_SUBLATTICE_SPLIT 1
 
-/* scoring kernel selectors for sqsg_score_batch / sqsg_job_set_kernel */
+/* scoring kernel selectors for sqsg_score_batch / sqsg_job_set_kernel.  All variants give
+ * identical results (bit-exact bonds, objectives and result packs).  AUTO runs the dense
+ * kernel where it applies (one sublattice problem, N <= 1024, K <= 16, at most 127 upper
+ * neighbours per site and shell), else the lane kernel (K <= 8), else the wave kernel;
+ * explicit scoring (sqsg_score_batch) with AUTO uses the wave kernel. */
 #define SQSG_KERNEL_AUTO 0
 #define SQSG_KERNEL_SPARSE_WAVE 1   /* one wavefront per candidate, LDS pair histogram */
 #define SQSG_KERNEL_SPARSE_LANE 2   /* one lane per candidate, LDS [bin][lane] counters */
@@ -228,11 +232,14 @@ typedef struct sqsg_server_options {
 } sqsg_server_options;
 SQSG_API int sqsg_server_create(const sqsg_server_options *opt, sqsg_server **out);
 /* Queues a created job; returns a ticket.  The server owns nothing: the caller keeps the
- * job and must not touch it until sqsg_server_wait reports it done. */
+ * job and must not touch it until sqsg_server_poll reports it done. */
 SQSG_API int sqsg_server_submit(sqsg_server *srv, sqsg_job *job, uint64_t *ticket);
-/* Runs scheduling steps until job `ticket` finished (or all, for ticket == UINT64_MAX);
- * returns its admission and completion times in seconds since server creation. */
+/* One scheduling step: admits queued jobs while the footprint and resident-job limits
+ * allow, then advances every resident job by one persistent launch and retires finished
+ * ones; reports the resident and queued job counts after the step. */
 SQSG_API int sqsg_server_step(sqsg_server *srv, int *active_jobs, int *queued_jobs);
+/* Status of a ticket: done flag and submission, admission and completion times in seconds
+ * since server creation (-1 where not reached yet). */
 SQSG_API int sqsg_server_poll(sqsg_server *srv, uint64_t ticket, int *done, double *t_submit,
                               double *t_admit, double *t_done);
 SQSG_API void sqsg_server_destroy(sqsg_server *srv);

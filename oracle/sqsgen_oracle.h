@@ -12,7 +12,30 @@
  * reference's atom_pair<usize_t> (core/structure.h:21-27), bond counters are
  * 64-bit like cube_t<usize_t>.
  *
- * Pinning status (what checks each function) is listed in oracle/README.md.
+ * Pinning status (tests/test_oracle.py unless noted):
+ *   or_rapid_mix           golden vectors of the reference's own rapid_mix, compiled from
+ *                          rapidhash.h into oracle/_ref (tests/golden/rapid_mix.json)
+ *   or_rapidrand           golden draw streams from the same build; counter property
+ *   or_random_bounded      formula and range; Fisher-Yates uniformity (chi-square) and
+ *   or_shuffle_random      window independence.  The exact trajectory of a reference MC
+ *                          run (threads sharing the RNG state, block split of the thread
+ *                          pool, std::sort site order) is PARITY UNPINNED: MC results are
+ *                          checked by re-scoring instead.
+ *   or_num/rank/unrank/next_permutation  known answers (C(32,16), first/last ranks),
+ *                          brute-force lexicographic positions, unrank/rank round trip
+ *   or_distance_matrix, or_shell_matrix, or_supercell  the reference's fixture case1
+ *                          (its tests/assets data, in tests/golden/case1.*.json); FCC
+ *                          coordination numbers
+ *   or_distances_naive     FCC 3x3x3 naive distances and coordination numbers
+ *   or_distances_histogram equals the naive radii on perfect FCC 2x2x2 / 3x3x3 lattices
+ *                          (no golden vector of the reference's own histogram exists)
+ *   or_compute_prefactors, or_scaled_pair_weights  closed forms
+ *   or_count_bonds         brute-force dense contraction X^T U X; per-shell totals = pairs
+ *   or_compute_objective   closed forms, zero weights, unit prefactors, summation order
+ *   or_run                 systematic: all 12870 results with zero pair weights; keep
+ *                          levels hold every tie; max_results_per_objective allows max+1.
+ *                          Membership of non-optimal MC levels is PARITY UNPINNED (it
+ *                          depends on the unpinned trajectory).
  */
 #ifndef SQSGEN_ORACLE_H
 #define SQSGEN_ORACLE_H
