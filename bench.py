@@ -272,7 +272,7 @@ def main() -> int:
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS,
                      "traffic": measured_traffic(per_launch),
-                     "kernel": "batch_dense_kernel<8,3,2>", "avg_launch_ms": avg_launch_s * 1e3,
+                     "kernel": "batch_dense_kernel<8,2,2,12>", "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_ops_per_candidate": ops_per_cand,
                      "candidates_per_launch": per_launch,
                      "note": "int8 ops (2 per multiply-accumulate) of the dense one-hot "

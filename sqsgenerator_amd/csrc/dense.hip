@@ -69,32 +69,27 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T &constant_r
   return ((const __attribute__((address_space(4))) T *)p)[i];
 }
 
-// Tile-stream prefetch depth (in grid positions) of a tile bound; TT (TT + 1) / 2 grid
-// positions per shell must be a multiple of it so every shell reuses the same ring slots.
-template <int TT> struct DenseRing {
-  static constexpr int P = TT * (TT + 1) / 2;
-  static constexpr int D = TT >= 32 ? 16 : TT >= 28 ? 14 : TT >= 16 ? 8 : TT >= 8 ? 6
-                         : TT >= 4 ? 5 : P;
-  static_assert(P % D == 0, "ring depth must divide the grid positions of a shell");
-};
 
 // launch shapes: tile bound TT, 32-column tile groups G per wavefront (X holds G * TT
-// one-hot tiles in registers) and resident wavefronts W per SIMD the registers are
-// budgeted for.  With two per SIMD (256 registers each), one wavefront generates candidates
-// or sums objectives while the other keeps the matrix core busy; large tile bounds need the
-// whole register file for the one-hot tiles and the prefetch ring.
-template <int TT_, int G_, int W_> struct DenseShape {
-  static constexpr int TT = TT_, G = G_, W = W_;
+// one-hot tiles in registers), resident wavefronts W per SIMD the registers are budgeted
+// for, and the tile-stream prefetch depth D (grid positions; it must divide the
+// TT (TT + 1) / 2 positions of a shell so every shell reuses the same ring slots).  With
+// two wavefronts per SIMD (256 registers each), one generates candidates or sums
+// objectives while the other keeps the matrix core busy; large tile bounds need the whole
+// register file for the one-hot tiles and the ring.
+template <int TT_, int G_, int W_, int D_> struct DenseShape {
+  static constexpr int TT = TT_, G = G_, W = W_, D = D_;
+  static_assert((TT * (TT + 1) / 2) % D == 0, "ring depth must divide a shell's positions");
 };
 template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
   switch (tile_bound) {
-    case 1: return f(DenseShape<1, 4, 2>{});
-    case 2: return f(DenseShape<2, 4, 2>{});
-    case 4: return f(DenseShape<4, 2, 2>{});
-    case 8: return f(DenseShape<8, 3, 2>{});
-    case 16: return f(DenseShape<16, 2, 1>{});
-    case 28: return f(DenseShape<28, 1, 1>{});
-    case 32: return f(DenseShape<32, 1, 1>{});
+    case 1: return f(DenseShape<1, 4, 2, 1>{});
+    case 2: return f(DenseShape<2, 4, 2, 3>{});
+    case 4: return f(DenseShape<4, 2, 2, 5>{});
+    case 8: return f(DenseShape<8, 2, 2, 12>{});
+    case 16: return f(DenseShape<16, 2, 1, 8>{});
+    case 28: return f(DenseShape<28, 1, 1, 14>{});
+    case 32: return f(DenseShape<32, 1, 1, 16>{});
     default: return hipErrorInvalidValue;
   }
 }
@@ -109,15 +104,15 @@ __device__ __forceinline__ uint32_t tile_offset(uint32_t s, uint32_t I, uint32_t
 }
 
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
-// bytes, padding sites 0xff); `dump` is G * 1024 words of LDS.  Passes whose bit in `live`
+// bytes, padding sites 0xff); `cnt` is CW * K * K words of LDS.  Passes whose bit in `live`
 // is clear are skipped.  Returns the objective of the lane's own candidate
 // (lane < npass * CW) and, when given, writes its ordered bonds and SRO cube.
-template <int TT, int G, class PR>
+template <int TT, int G, int D, class PR>
 __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
-                                              uint32_t stride, uint32_t *dump, uint32_t CW,
+                                              uint32_t stride, uint32_t *cnt, uint32_t CW,
                                               uint32_t npass, unsigned long long live, int lane,
                                               int32_t *bonds, double *sro) {
-  constexpr int D = DenseRing<TT>::D, P = DenseRing<TT>::P;
+  constexpr int P = TT * (TT + 1) / 2;
   const uint32_t K = pr.K, KK = K * K, T = pr.dT, S = pr.S;
   const uint32_t g = CW / G;
   const uint32_t h = static_cast<uint32_t>(lane) >> 5, n = static_cast<uint32_t>(lane) & 31u;
@@ -130,7 +125,12 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   const uint32_t toff = static_cast<uint32_t>(lane) * 16u;
   const uint32_t my_pass = static_cast<uint32_t>(lane) / CW;
   const uint32_t my_cc = static_cast<uint32_t>(lane) - my_pass * CW;  // candidate in its pass
-  const uint32_t my_group = my_cc / g, my_c = my_cc - my_group * g;
+  uint32_t keep = 0;  // accumulator registers of GEMM2 on this lane's diagonal block
+#pragma unroll
+  for (uint32_t r = 0; r < 16; ++r) {
+    const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
+    if (col_ok && m / K == cl) keep |= 1u << r;
+  }
   double obj = 0.0;
   for (uint32_t pass = 0; pass < npass; ++pass) {
     if (!((live >> pass) & 1ull)) continue;
@@ -203,22 +203,20 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
         }
         if (++J == TT) ++I, J = I;
       }
-      // accumulators -> LDS [group][lane][register]; the diagonal blocks are read below
+      // diagonal blocks -> counts [candidate of the pass][a][b]: register r of lane l is
+      // row m = 8(r/4) + 4h + r%4 = cK + a of column n = cK + b
 #pragma unroll
       for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-        for (int w = 0; w < 4; ++w)
-          *reinterpret_cast<uint4 *>(dump + (gi * 64 + lane) * 16 + 4 * w) =
-              make_uint4(C[gi][4 * w], C[gi][4 * w + 1], C[gi][4 * w + 2], C[gi][4 * w + 3]);
+        for (uint32_t r = 0; r < 16; ++r)
+          if ((keep >> r) & 1u) {
+            const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
+            cnt[gi * g * KK + K * m + b] = static_cast<uint32_t>(C[gi][r]);
+          }
       wave_sync();
       if (my_pass == pass) {
-        // count (a, b) of candidate c of group gi: accumulator row m = cK + a, column
-        // n = cK + b, held by lane 32 ((m / 4) % 2) + n in register 4 (m / 8) + m % 4
-        const uint32_t *dg = dump + my_group * 1024u;
-        auto count = [&](uint32_t a, uint32_t b2) {
-          const uint32_t m = my_c * K + a, nn = my_c * K + b2;
-          return dg[(32 * ((m >> 2) & 1u) + nn) * 16 + 4 * (m >> 3) + (m & 3u)];
-        };
+        const uint32_t *cc = cnt + my_cc * KK;
+        auto count = [&](uint32_t a, uint32_t b2) { return cc[a * K + b2]; };
         const uint32_t o = s * KK;
         for (uint32_t xi = 0; xi < K; ++xi) {
           const uint32_t cxx = count(xi, xi);
@@ -254,7 +252,7 @@ template <class PR> __device__ __forceinline__ void load_dense_row(uint8_t *row,
 // One segment of a job: gen_group lanes x p.rounds candidates starting at sbeg (the
 // candidate numbering of the lane kernel with 64 replaced by gen_group); candidates at or
 // beyond `end` are inactive.
-template <int TT, int G, class RP>
+template <int TT, int G, int D, class RP>
 __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64_t end,
                                               uint8_t *wl, int lane, LevelFilter &lf,
                                               unsigned long long &scored) {
@@ -262,7 +260,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
   const uint32_t GEN = p.gen_group, CW = p.dense_cw, stride = p.dense_stride;
   const uint32_t npass = GEN / CW;
   uint8_t *rows = wl + p.off_rows[0];
-  uint32_t *dump = reinterpret_cast<uint32_t *>(wl + p.off_hist);
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(wl + p.off_hist);
   const bool gl = static_cast<uint32_t>(lane) < GEN;
   uint8_t *row = rows + (gl ? static_cast<uint32_t>(lane) : 0u) * stride;
   for (uint32_t r = 0; r < p.rounds; ++r) {
@@ -290,7 +288,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
       if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
     wave_sync();
     const double tot =
-        dense_score<TT, G>(pr, rows, stride, dump, CW, npass, live, lane, nullptr, nullptr);
+        dense_score<TT, G, D>(pr, rows, stride, cnt, CW, npass, live, lane, nullptr, nullptr);
     scored += static_cast<unsigned long long>(__popcll(am));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -314,7 +312,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
   }
 }
 
-template <int TT, int G, int W>
+template <int TT, int G, int W, int D>
 __global__ __launch_bounds__(64, W) void run_dense_kernel(RunParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
@@ -331,14 +329,14 @@ __global__ __launch_bounds__(64, W) void run_dense_kernel(RunParams p) {
     if (lane == 0) seg = atomicAdd(p.next_seg, 1u);
     seg = __builtin_amdgcn_readfirstlane(__shfl(seg, 0));
     if (seg >= nseg) break;
-    dense_segment<TT, G>(p, p.cand_begin + seg * per_seg, p.cand_end, lds, lane, lf, scored);
+    dense_segment<TT, G, D>(p, p.cand_begin + seg * per_seg, p.cand_end, lds, lane, lf, scored);
   }
   if (lane == 0 && scored) atomicAdd(p.scored, scored);
 }
 
 // Persistent form: every wavefront claims one-segment chunks of the jobs' candidate ranges
 // until the work list is drained; the jobs of a launch share the tile bound TT.
-template <int TT, int G, int W>
+template <int TT, int G, int W, int D>
 __global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
@@ -359,19 +357,19 @@ __global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
       lf.n = 0;
       last_job = job;
     }
-    dense_segment<TT, G>(p, cbeg, cend, lds, lane, lf, scored);
+    dense_segment<TT, G, D>(p, cbeg, cend, lds, lane, lf, scored);
   }
   if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
 }
 
-template <int TT, int G>
+template <int TT, int G, int D>
 __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
   const DevProblem &pr = p.prob;
   const uint32_t GEN = p.dense_gen, CW = p.dense_cw, npass = GEN / CW;
   uint8_t *rows = lds + p.off_row;
-  uint32_t *dump = reinterpret_cast<uint32_t *>(lds + p.off_hist);
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(lds + p.off_hist);
   const uint32_t nb = pr.S * pr.K * pr.K;
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * GEN + lane;
   const bool gl = static_cast<uint32_t>(lane) < GEN;
@@ -386,7 +384,7 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   for (uint32_t q = 0; q < npass; ++q)
     if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
   wave_sync();
-  const double obj = dense_score<TT, G>(pr, rows, p.dense_stride, dump, CW, npass, live, lane,
+  const double obj = dense_score<TT, G, D>(pr, rows, p.dense_stride, cnt, CW, npass, live, lane,
                                         active && p.bonds ? p.bonds + b * nb : nullptr,
                                         active && p.sro ? p.sro + b * nb : nullptr);
   if (active) p.objective[b] = obj;
@@ -398,28 +396,29 @@ hipError_t dense_lds(const void *fn, size_t bytes) {
                              static_cast<int>(bytes));
 }
 
-template <int TT, int G, int W> hipError_t run_dense_k(const RunParams &p, uint32_t blocks,
-                                                       hipStream_t s) {
-  const hipError_t e = dense_lds(reinterpret_cast<const void *>(&run_dense_kernel<TT, G, W>),
-                                 p.wave_lds);
+template <class Sh> hipError_t run_dense_k(const RunParams &p, uint32_t blocks, hipStream_t s) {
+  const void *fn = reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>);
+  const hipError_t e = dense_lds(fn, p.wave_lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((run_dense_kernel<TT, G, W>), dim3(blocks), dim3(64), p.wave_lds, s, p);
+  hipLaunchKernelGGL((run_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>), dim3(blocks), dim3(64),
+                     p.wave_lds, s, p);
   return hipGetLastError();
 }
 
-template <int TT, int G, int W> hipError_t batch_dense_k(const BatchParams &bp, uint32_t blocks,
-                                                         hipStream_t s) {
-  hipLaunchKernelGGL((batch_dense_kernel<TT, G, W>), dim3(blocks), dim3(64), bp.wave_lds, s,
-                     bp);
+template <class Sh> hipError_t batch_dense_k(const BatchParams &bp, uint32_t blocks,
+                                              hipStream_t s) {
+  hipLaunchKernelGGL((batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>), dim3(blocks), dim3(64),
+                     bp.wave_lds, s, bp);
   return hipGetLastError();
 }
 
-template <int TT, int G> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s) {
-  const hipError_t e = dense_lds(reinterpret_cast<const void *>(&score_dense_kernel<TT, G>),
-                                 p.wave_lds);
+template <class Sh> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s) {
+  const void *fn = reinterpret_cast<const void *>(&score_dense_kernel<Sh::TT, Sh::G, Sh::D>);
+  const hipError_t e = dense_lds(fn, p.wave_lds);
   if (e != hipSuccess) return e;
   const uint32_t blocks = static_cast<uint32_t>((p.B + p.dense_gen - 1) / p.dense_gen);
-  hipLaunchKernelGGL((score_dense_kernel<TT, G>), dim3(blocks), dim3(64), p.wave_lds, s, p);
+  hipLaunchKernelGGL((score_dense_kernel<Sh::TT, Sh::G, Sh::D>), dim3(blocks), dim3(64),
+                     p.wave_lds, s, p);
   return hipGetLastError();
 }
 
@@ -444,8 +443,8 @@ uint32_t dense_waves_per_cu(uint32_t tile_bound, size_t wave_lds, bool persisten
   const hipError_t e = with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
     const void *fn = persistent
-                         ? reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W>)
-                         : reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G, Sh::W>);
+                         ? reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>)
+                         : reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>);
     hipError_t r = dense_lds(fn, wave_lds);
     if (r != hipSuccess) return r;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, wave_lds);
@@ -457,14 +456,15 @@ hipError_t launch_run_dense(const RunParams &p, uint32_t tile_bound, uint32_t bl
                             hipStream_t s) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return run_dense_k<Sh::TT, Sh::G, Sh::W>(p, blocks, s);
+    return run_dense_k<Sh>(p, blocks, s);
   });
 }
 
 hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return dense_lds(reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W>), lds);
+    return dense_lds(
+        reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>), lds);
   });
 }
 
@@ -472,7 +472,7 @@ hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32
                               hipStream_t s) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return batch_dense_k<Sh::TT, Sh::G, Sh::W>(bp, blocks, s);
+    return batch_dense_k<Sh>(bp, blocks, s);
   });
 }
 
@@ -480,7 +480,7 @@ hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStre
   if (p.B == 0) return hipSuccess;
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return score_dense_k<Sh::TT, Sh::G>(p, s);
+    return score_dense_k<Sh>(p, s);
   });
 }
 

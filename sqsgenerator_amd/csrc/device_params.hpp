@@ -60,7 +60,7 @@ struct RunParams {
   // LDS layout of one wave (bytes)
   uint32_t wave_lds;
   uint32_t off_rows[kMaxProblems];
-  uint32_t off_hist, off_terms, off_obj;  // dense: off_hist holds the accumulator dump
+  uint32_t off_hist, off_terms, off_obj;  // dense: off_hist holds one shell's counts
   // tie-keeping threshold and record sink
   unsigned long long *thr_key;
   unsigned int *rec_count;

@@ -505,13 +505,13 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   uint32_t off = 0;
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
-    // rows of gen_group candidates (npass passes of dense_cw) + the accumulator dump
+    // rows of gen_group candidates (npass passes of dense_cw) + one shell's counts
     const Problem &P = job->problems[0];
     const uint32_t T = (P.N + 31) / 32, G = dense_groups(dense_tile_bound(T));
     const uint32_t g = std::min(32 / P.K, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
-    const uint32_t cnt_bytes = G * 4096;  // accumulator dump [G][64 lanes][16]
+    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4);  // one shell's counts
     uint32_t npass = 64 / p.dense_cw;
     // at most 20 KiB per wavefront: eight resident wavefronts per CU (two per SIMD)
     while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 20 * 1024) --npass;
