@@ -7,8 +7,10 @@
 // Jobs are grouped by kernel variant and its specialisation (largest species count for the
 // lane kernel, tile bound for the dense MFMA kernel); each group has fixed device tables
 // and a captured graph {counter reset kernel, persistent kernel} that is replayed every
-// step.  Between steps the host drains the record sinks, tightens each job's threshold and
-// admits or retires jobs, which is what the serving loop builds on.
+// step.  Steps are pipelined two deep: each job's record sink and counters are
+// double-buffered by step parity, so while step k runs the host reads back step k - 1,
+// merges its records, tightens each job's threshold and admits or retires jobs (which is
+// what the serving loop builds on); the device never waits for the host between steps.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -16,6 +18,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 
 #include "job.hpp"
 #include "kernels.hpp"
@@ -24,24 +27,58 @@ namespace sqsg {
 
 namespace {
 
-constexpr uint32_t kGroupJobs = 256;      // resident jobs per species-count group
-constexpr uint32_t kGroupItems = kGroupJobs;  // one work item per resident job and step
+constexpr uint32_t kGroupJobs = 256;          // resident jobs per group
+constexpr uint32_t kGroupItems = 2 * kGroupJobs;  // a job's re-issued range + its next slice
 
 struct Group;
+
+// Host copies of a slot's device counters, in pinned memory so the copies are asynchronous.
+struct HostCounters {
+  unsigned long long scored[2];
+  unsigned long long thr;
+  unsigned int count[2];
+  unsigned int pad[2];
+};
+
+using Range = std::pair<uint64_t, uint64_t>;
+
+// Device buffers of a resident job, pooled by the engine across jobs: record sinks and
+// counters double-buffered per step parity (a step writes one set while the host reads the
+// other), the threshold, and pinned host copies of the counters.
+struct SlotBuffers {
+  RecordSink sink[2];
+  DevBuf count[2], scored[2];
+  DevBuf thr;
+  HostCounters *hc = nullptr;
+  SlotBuffers() {
+    for (int q = 0; q < 2; ++q) {
+      count[q].alloc(sizeof(unsigned int));
+      scored[q].alloc(sizeof(unsigned long long));
+    }
+    thr.alloc(sizeof(unsigned long long));
+    void *h = nullptr;
+    SQSG_HIP(hipHostMalloc(&h, sizeof(HostCounters), hipHostMallocDefault));
+    hc = static_cast<HostCounters *>(h);
+  }
+  ~SlotBuffers() {
+    if (hc) (void)hipHostFree(hc);
+  }
+};
 
 struct Slot {
   sqsg_job *job = nullptr;
   Group *group = nullptr;
   RunParams p{};
-  RecordSink sink;
-  DevBuf thr, count, scored;
-  uint64_t next = 0;         // first candidate of the next slice
-  uint64_t slice = 0;        // candidates per step
-  uint64_t chunk = 0;        // candidates a wavefront claims at once (one segment)
-  uint64_t issued_end = 0;   // end of the slice issued in the current step
-  unsigned int h_count = 0;  // host copies of the device counters after a step
-  unsigned long long h_scored = 0, h_thr = 0;
-  size_t footprint = 0;      // device bytes held
+  std::unique_ptr<SlotBuffers> buf;
+  RecordSink *sink = nullptr;   // buf->sink
+  DevBuf *count = nullptr, *scored = nullptr;
+  HostCounters *hc = nullptr;
+  uint64_t next = 0;          // first candidate not issued yet
+  uint64_t slice = 0;         // candidates per step
+  uint64_t chunk = 0;         // candidates a wavefront claims at once (one segment)
+  std::vector<Range> issued[2];  // ranges of the step of each parity
+  std::deque<Range> retry;    // ranges to issue again (their record sink overflowed)
+  size_t footprint = 0;       // device bytes held
   bool done = false;
 };
 
@@ -52,13 +89,31 @@ struct Group {
   hipGraphExec_t exec = nullptr;
   uint32_t blocks = 0;     // of the current step's launch
   BatchParams bp{};
-  std::vector<RunParams> h_table;  // host sources of the step's uploads
-  std::vector<WorkItem> h_items;
-  uint32_t h_counts[2] = {0, 0};
+  // pinned host sources of each parity's uploads (asynchronous copies, and the buffer of
+  // a step stays untouched until that step was processed)
+  RunParams *h_table[2] = {nullptr, nullptr};
+  WorkItem *h_items[2] = {nullptr, nullptr};
+  uint32_t *h_counts[2] = {nullptr, nullptr};
   uint32_t graph_wave_lds = 0, graph_blocks = 0;
   std::vector<Slot *> members;
+  Group() {
+    for (int q = 0; q < 2; ++q) {
+      void *t = nullptr, *i = nullptr, *c = nullptr;
+      SQSG_HIP(hipHostMalloc(&t, kGroupJobs * sizeof(RunParams), hipHostMallocDefault));
+      h_table[q] = static_cast<RunParams *>(t);
+      SQSG_HIP(hipHostMalloc(&i, kGroupItems * sizeof(WorkItem), hipHostMallocDefault));
+      h_items[q] = static_cast<WorkItem *>(i);
+      SQSG_HIP(hipHostMalloc(&c, 2 * sizeof(uint32_t), hipHostMallocDefault));
+      h_counts[q] = static_cast<uint32_t *>(c);
+    }
+  }
   ~Group() {
     if (exec) (void)hipGraphExecDestroy(exec);
+    for (int q = 0; q < 2; ++q) {
+      if (h_table[q]) (void)hipHostFree(h_table[q]);
+      if (h_items[q]) (void)hipHostFree(h_items[q]);
+      if (h_counts[q]) (void)hipHostFree(h_counts[q]);
+    }
   }
 };
 
@@ -76,13 +131,24 @@ class BatchEngine {
  public:
   explicit BatchEngine(bool use_graph) : use_graph_(use_graph) {
     SQSG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    SQSG_HIP(hipEventCreate(&ev0_));
-    SQSG_HIP(hipEventCreate(&ev1_));
+    SQSG_HIP(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking));
+    for (int q = 0; q < 2; ++q) {
+      SQSG_HIP(hipEventCreate(&ev_start_[q]));
+      SQSG_HIP(hipEventCreate(&ev_done_[q]));
+    }
   }
   ~BatchEngine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
     groups_.clear();
-    if (ev0_) (void)hipEventDestroy(ev0_);
-    if (ev1_) (void)hipEventDestroy(ev1_);
+    slots_.clear();
+    retired_[0].clear();
+    retired_[1].clear();
+    pool_.clear();
+    for (int q = 0; q < 2; ++q) {
+      if (ev_start_[q]) (void)hipEventDestroy(ev_start_[q]);
+      if (ev_done_[q]) (void)hipEventDestroy(ev_done_[q]);
+    }
+    if (copy_) (void)hipStreamDestroy(copy_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -112,17 +178,24 @@ class BatchEngine {
     s->chunk = gen * s->p.rounds;
     s->slice = std::max(s->slice, s->chunk);
     s->p.rec_stride = job->stride;
-    s->sink.ensure(static_cast<uint32_t>(std::min<uint64_t>(s->slice, 1u << 14)), job->stride,
-                   s->p.nprob);
-    s->thr.alloc(sizeof(unsigned long long));
-    s->count.alloc(sizeof(unsigned int));
-    s->scored.alloc(sizeof(unsigned long long));
-    const unsigned long long inf = objective_key(std::numeric_limits<double>::infinity());
-    SQSG_HIP(hipMemcpy(s->thr.as<void>(), &inf, sizeof(inf), hipMemcpyHostToDevice));
-    s->p.thr_key = s->thr.as<unsigned long long>();
-    s->p.rec_count = s->count.as<unsigned int>();
-    s->p.scored = s->scored.as<unsigned long long>();
-    s->footprint = job_footprint(job) + s->sink.cap * (24 + job->stride);
+    if (!pool_.empty()) {
+      s->buf = std::move(pool_.back());
+      pool_.pop_back();
+    } else {
+      s->buf = std::make_unique<SlotBuffers>();
+    }
+    s->sink = s->buf->sink;
+    s->count = s->buf->count;
+    s->scored = s->buf->scored;
+    s->hc = s->buf->hc;
+    const uint32_t cap = static_cast<uint32_t>(std::min<uint64_t>(s->slice, 1u << 14));
+    for (int q = 0; q < 2; ++q) s->sink[q].ensure(cap, job->stride, s->p.nprob);
+    *s->hc = HostCounters{};
+    s->hc->thr = objective_key(std::numeric_limits<double>::infinity());
+    SQSG_HIP(hipMemcpyAsync(s->buf->thr.as<void>(), &s->hc->thr, sizeof(s->hc->thr),
+                            hipMemcpyHostToDevice, stream_));
+    s->p.thr_key = s->buf->thr.as<unsigned long long>();
+    s->footprint = footprint(*s);
     Group &g = group(variant, variant == SQSG_KERNEL_DENSE_MFMA ? dense_tile_bound_of(job)
                                                                 : job->kmax);
     if (g.members.size() >= kGroupJobs) throw std::out_of_range("too many resident jobs");
@@ -133,88 +206,40 @@ class BatchEngine {
     return fp;
   }
 
-  bool idle() const { return slots_.empty(); }
+  bool idle() const { return slots_.empty() && !in_flight_[0] && !in_flight_[1]; }
+  // releases retired slots (only when idle: no issued table names them any more)
+  void release_retired() {
+    if (!idle()) return;
+    recycle(retired_[0]);
+    recycle(retired_[1]);
+  }
   size_t resident() const { return slots_.size(); }
 
-  // One persistent launch over every active job; finished jobs are finalised and returned.
+  // Issues the next persistent step (when any resident job has work left), then processes
+  // the previous step while the new one runs: counters, record sinks, thresholds,
+  // retirement.  Finished jobs are finalised and returned.
   std::vector<sqsg_job *> step() {
-    std::vector<sqsg_job *> finished;
-    if (slots_.empty()) return finished;
-    const auto t0 = std::chrono::steady_clock::now();
-    // tables first, then the launches bracketed by the step's events
-    std::vector<Group *> live;
-    for (auto &[k, g] : groups_)
-      if (!g->members.empty()) live.push_back(prepare_group(*g));
-    SQSG_HIP(hipEventRecord(ev0_, stream_));
-    for (Group *g : live) launch_group(*g);
-    SQSG_HIP(hipEventRecord(ev1_, stream_));
-    SQSG_HIP(hipStreamSynchronize(stream_));
-    float ms = 0;
-    SQSG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-    kernel_ms += ms;
-    ++launches;
-    // every slot's counters and threshold in one round trip
-    for (auto &s : slots_) {
-      if (s->issued_end == s->next) continue;
-      SQSG_HIP(hipMemcpyAsync(&s->h_count, s->count.as<void>(), sizeof(s->h_count),
-                              hipMemcpyDeviceToHost, stream_));
-      SQSG_HIP(hipMemcpyAsync(&s->h_scored, s->scored.as<void>(), sizeof(s->h_scored),
-                              hipMemcpyDeviceToHost, stream_));
-      SQSG_HIP(hipMemcpyAsync(&s->h_thr, s->thr.as<void>(), sizeof(s->h_thr),
-                              hipMemcpyDeviceToHost, stream_));
-    }
-    SQSG_HIP(hipStreamSynchronize(stream_));
-    for (auto &s : slots_) {
-      sqsg_job *job = s->job;
-      if (s->issued_end == s->next) {
-        if (s->next >= job->iterations || job->cancel.load()) s->done = true;
-        continue;
-      }
-      const unsigned int count = s->h_count;
-      const unsigned long long scored = s->h_scored;
-      job->launches++;
-      job->kernel_ms += ms;  // shared launch: every resident job saw the whole step
-      if (count > s->sink.cap) {  // widen and issue the same slice again next step
-        s->sink.ensure(count + count / 4, job->stride, s->p.nprob);
-        s->footprint = job_footprint(job) + s->sink.cap * (24 + job->stride);
-        continue;
-      }
-      if (scored != s->issued_end - s->next)
-        throw DeviceError("persistent launch scored an unexpected candidate count");
-      job->scored += scored;
-      candidates += scored;
-      s->sink.drain(job, count, s->p.nprob, stream_);
-      // keep the device threshold at the tighter of its own bound and the exact host one
-      // (ordered before the next launch on the same stream; h_thr outlives the copy)
-      const unsigned long long hk = objective_key(job->vkeep);
-      if (hk < s->h_thr) {
-        s->h_thr = hk;
-        SQSG_HIP(hipMemcpyAsync(s->thr.as<void>(), &s->h_thr, sizeof(s->h_thr),
-                                hipMemcpyHostToDevice, stream_));
-      }
-      s->next = s->issued_end;
-      job->stats.finished = job->scored;
-      if (s->next >= job->iterations || job->cancel.load()) s->done = true;
-    }
-    const auto t1 = std::chrono::steady_clock::now();
-    for (auto it = slots_.begin(); it != slots_.end();) {
-      if (!(*it)->done) {
-        ++it;
-        continue;
-      }
-      sqsg_job *job = (*it)->job;
-      finalize_results(job);
-      job->stats.timing_loop_ns +=
-          std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
-      job->stats.timing_total_ns = job->stats.timing_loop_ns;
-      Group &g = *(*it)->group;
-      g.members.erase(std::find(g.members.begin(), g.members.end(), it->get()));
-      finished.push_back(job);
-      it = slots_.erase(it);
-    }
+    const int p = parity_;
+    parity_ ^= 1;
+    recycle(retired_[p]);  // the last table that listed them was parity p's previous step
+    bool work = false;
     for (auto &s : slots_)
-      s->job->stats.timing_loop_ns +=
-          std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+      work |= !s->retry.empty() || (s->next < s->job->iterations && !s->job->cancel.load());
+    if (work) {
+      const auto t0 = std::chrono::steady_clock::now();
+      std::vector<Group *> live;
+      for (auto &[k, g] : groups_)
+        if (!g->members.empty()) live.push_back(prepare_group(*g, p));
+      SQSG_HIP(hipEventRecord(ev_start_[p], stream_));
+      for (Group *g : live) launch_group(*g);
+      SQSG_HIP(hipEventRecord(ev_done_[p], stream_));
+      in_flight_[p] = true;
+      host_ns_ += std::chrono::duration_cast<std::chrono::nanoseconds>(
+                      std::chrono::steady_clock::now() - t0).count();
+    }
+    // the previous call processed the step before that, so only parity p ^ 1 can be pending
+    std::vector<sqsg_job *> finished;
+    if (in_flight_[p ^ 1]) process(p ^ 1, finished);
     return finished;
   }
 
@@ -223,6 +248,91 @@ class BatchEngine {
   uint32_t last_items = 0;
 
  private:
+  // returns the buffers of retired slots to the pool
+  void recycle(std::vector<std::unique_ptr<Slot>> &slots) {
+    for (auto &s : slots) pool_.push_back(std::move(s->buf));
+    slots.clear();
+  }
+
+  size_t footprint(const Slot &s) const {
+    return job_footprint(s.job) + (s.sink[0].cap + s.sink[1].cap) * (24 + s.job->stride);
+  }
+
+  // Reads back the step of parity q (issued before the one now running) and applies it.
+  void process(int q, std::vector<sqsg_job *> &finished) {
+    const auto t0 = std::chrono::steady_clock::now();
+    SQSG_HIP(hipEventSynchronize(ev_done_[q]));
+    in_flight_[q] = false;
+    float ms = 0;
+    SQSG_HIP(hipEventElapsedTime(&ms, ev_start_[q], ev_done_[q]));
+    kernel_ms += ms;
+    ++launches;
+    for (auto &s : slots_) {
+      if (s->issued[q].empty()) continue;
+      SQSG_HIP(hipMemcpyAsync(&s->hc->count[q], s->count[q].as<void>(), sizeof(unsigned int),
+                              hipMemcpyDeviceToHost, copy_));
+      SQSG_HIP(hipMemcpyAsync(&s->hc->scored[q], s->scored[q].as<void>(),
+                              sizeof(unsigned long long), hipMemcpyDeviceToHost, copy_));
+      SQSG_HIP(hipMemcpyAsync(&s->hc->thr, s->buf->thr.as<void>(), sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, copy_));
+    }
+    SQSG_HIP(hipStreamSynchronize(copy_));
+    for (auto &s : slots_) {
+      sqsg_job *job = s->job;
+      if (!s->issued[q].empty()) {
+        uint64_t expect = 0;
+        for (const Range &r : s->issued[q]) expect += r.second - r.first;
+        const unsigned int count = s->hc->count[q];
+        job->launches++;
+        job->kernel_ms += ms;  // shared launch: every resident job saw the whole step
+        if (count > s->sink[q].cap) {
+          // the sink kept only part of the records: widen it and issue the ranges again
+          s->sink[q].ensure(count + count / 4, job->stride, s->p.nprob);
+          s->footprint = footprint(*s);
+          for (const Range &r : s->issued[q]) s->retry.push_back(r);
+        } else {
+          if (s->hc->scored[q] != expect)
+            throw DeviceError("persistent launch scored an unexpected candidate count");
+          job->scored += expect;
+          candidates += expect;
+          s->sink[q].drain(job, count, s->p.nprob, copy_);
+          // keep the device threshold at the tighter of its own bound and the exact host
+          // one; ordered before the next step (a step running now only lowers it further)
+          const unsigned long long hk = objective_key(job->vkeep);
+          if (hk < s->hc->thr) {
+            s->hc->thr = hk;
+            SQSG_HIP(hipMemcpyAsync(s->buf->thr.as<void>(), &s->hc->thr, sizeof(s->hc->thr),
+                                    hipMemcpyHostToDevice, stream_));
+          }
+          job->stats.finished = job->scored;
+        }
+        s->issued[q].clear();
+      }
+      const bool exhausted = s->next >= job->iterations || job->cancel.load();
+      if (exhausted && s->retry.empty() && s->issued[q ^ 1].empty()) s->done = true;
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    const int64_t ns =
+        std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count() + host_ns_;
+    host_ns_ = 0;
+    for (auto it = slots_.begin(); it != slots_.end();) {
+      (*it)->job->stats.timing_loop_ns += ns;
+      if (!(*it)->done) {
+        ++it;
+        continue;
+      }
+      sqsg_job *job = (*it)->job;
+      finalize_results(job);
+      job->stats.timing_total_ns = job->stats.timing_loop_ns;
+      Group &g = *(*it)->group;
+      g.members.erase(std::find(g.members.begin(), g.members.end(), it->get()));
+      finished.push_back(job);
+      // its counters stay allocated while an issued table may still name them
+      retired_[q ^ 1].push_back(std::move(*it));
+      it = slots_.erase(it);
+    }
+  }
+
   Group &group(int variant, uint32_t key) {
     auto &g = groups_[(static_cast<uint32_t>(variant) << 8) | key];
     if (!g) {
@@ -237,34 +347,42 @@ class BatchEngine {
     return *g;
   }
 
-  // uploads the group's job table and work items for this step
-  Group *prepare_group(Group &g) {
-    std::vector<RunParams> &table = g.h_table;
-    std::vector<WorkItem> &items = g.h_items;
-    table.resize(g.members.size());
-    items.clear();
+  // Builds and uploads the group's job table and work items for a step of parity q (the
+  // uploads are ordered after the step running now, which still reads the device tables)
+  Group *prepare_group(Group &g, int q) {
+    RunParams *table = g.h_table[q];
+    WorkItem *items = g.h_items[q];
+    uint32_t n = 0;
     uint32_t wave_lds = 0;
     for (uint32_t k = 0; k < g.members.size(); ++k) {
       Slot *s = g.members[k];
-      s->sink.bind(s->p);
+      s->sink[q].bind(s->p);
+      s->p.rec_count = s->count[q].as<unsigned int>();
+      s->p.scored = s->scored[q].as<unsigned long long>();
       table[k] = s->p;
       wave_lds = std::max(wave_lds, s->p.wave_lds);
-      const uint64_t end = std::min(s->job->iterations, s->next + s->slice);
-      s->issued_end = s->job->cancel.load() ? s->next : end;
-      if (s->issued_end > s->next)
-        items.push_back(WorkItem{k, static_cast<uint32_t>(s->chunk), s->next, s->issued_end});
+      s->issued[q].clear();
+      if (!s->retry.empty()) {
+        s->issued[q].push_back(s->retry.front());
+        s->retry.pop_front();
+      }
+      if (s->next < s->job->iterations && !s->job->cancel.load()) {
+        const uint64_t end = std::min(s->job->iterations, s->next + s->slice);
+        s->issued[q].push_back(Range{s->next, end});
+        s->next = end;
+      }
+      for (const Range &r : s->issued[q])
+        items[n++] = WorkItem{k, static_cast<uint32_t>(s->chunk), r.first, r.second};
     }
-    last_items = static_cast<uint32_t>(items.size());
-    const uint32_t n = last_items;
-    SQSG_HIP(hipMemcpyAsync(g.d_jobs.as<void>(), table.data(), table.size() * sizeof(RunParams),
+    last_items = n;
+    SQSG_HIP(hipMemcpyAsync(g.d_jobs.as<void>(), table, g.members.size() * sizeof(RunParams),
                             hipMemcpyHostToDevice, stream_));
     if (n)
-      SQSG_HIP(hipMemcpyAsync(g.d_items.as<void>(), items.data(), n * sizeof(WorkItem),
+      SQSG_HIP(hipMemcpyAsync(g.d_items.as<void>(), items, n * sizeof(WorkItem),
                               hipMemcpyHostToDevice, stream_));
-    g.h_counts[0] = n;
-    g.h_counts[1] = static_cast<uint32_t>(g.members.size());
-    // the host tables stay untouched until the step's end synchronisation
-    SQSG_HIP(hipMemcpyAsync(g.d_counts.as<void>(), g.h_counts, sizeof(g.h_counts),
+    g.h_counts[q][0] = n;
+    g.h_counts[q][1] = static_cast<uint32_t>(g.members.size());
+    SQSG_HIP(hipMemcpyAsync(g.d_counts.as<void>(), g.h_counts[q], 2 * sizeof(uint32_t),
                             hipMemcpyHostToDevice, stream_));
     const size_t block_lds = wave_lds;
     const bool dense = g.variant == SQSG_KERNEL_DENSE_MFMA;
@@ -320,24 +438,46 @@ class BatchEngine {
   }
 
   bool use_graph_;
-  hipStream_t stream_ = nullptr;
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipStream_t stream_ = nullptr;  // steps (and threshold updates)
+  hipStream_t copy_ = nullptr;    // read-back of finished steps
+  hipEvent_t ev_start_[2] = {nullptr, nullptr}, ev_done_[2] = {nullptr, nullptr};
+  bool in_flight_[2] = {false, false};
+  int parity_ = 0;
+  int64_t host_ns_ = 0;
   std::vector<std::unique_ptr<Slot>> slots_;
+  std::vector<std::unique_ptr<Slot>> retired_[2];
+  std::vector<std::unique_ptr<SlotBuffers>> pool_;
   std::map<uint32_t, std::unique_ptr<Group>> groups_;
 };
 
+// One engine per graph mode for the whole process: its groups, device tables and captured
+// graphs are reused by later calls.  It is never destroyed (the HIP runtime may already be
+// torn down when static destructors run).
 void batch_run(sqsg_job **jobs, uint32_t n, int use_graph, sqsg_batch_report *report) {
-  BatchEngine eng(use_graph != 0);
-  for (uint32_t k = 0; k < n; ++k) eng.add(jobs[k]);
-  uint32_t max_items = 0;
-  while (!eng.idle()) {
-    eng.step();
-    max_items = std::max(max_items, eng.last_items);
+  static std::mutex mutex;
+  static BatchEngine *engines[2] = {nullptr, nullptr};
+  std::lock_guard<std::mutex> lock(mutex);
+  BatchEngine *&eng = engines[use_graph ? 1 : 0];
+  if (!eng) eng = new BatchEngine(use_graph != 0);
+  const double ms0 = eng->kernel_ms;
+  const uint64_t c0 = eng->candidates, l0 = eng->launches;
+  try {
+    for (uint32_t k = 0; k < n; ++k) eng->add(jobs[k]);
+  } catch (...) {
+    while (!eng->idle()) eng->step();  // the jobs added so far finish normally
+    eng->release_retired();
+    throw;
   }
+  uint32_t max_items = 0;
+  while (!eng->idle()) {
+    eng->step();
+    max_items = std::max(max_items, eng->last_items);
+  }
+  eng->release_retired();
   if (report) {
-    report->kernel_ms = eng.kernel_ms;
-    report->candidates = eng.candidates;
-    report->launches = eng.launches;
+    report->kernel_ms = eng->kernel_ms - ms0;
+    report->candidates = eng->candidates - c0;
+    report->launches = eng->launches - l0;
     report->work_items = max_items;
   }
 }

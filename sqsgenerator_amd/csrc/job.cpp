@@ -565,12 +565,13 @@ hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, 
 }
 
 void RecordSink::ensure(uint32_t n, uint32_t stride, uint32_t nprob) {
-  if (n <= cap) return;
+  // buffers are kept when large enough (a pooled sink may come from a job of another shape)
+  if (cand.size() < size_t(n) * sizeof(uint64_t)) cand.alloc(size_t(n) * sizeof(uint64_t));
+  if (obj.size() < size_t(n) * sizeof(double)) obj.alloc(size_t(n) * sizeof(double));
+  if (sub.size() < size_t(n) * nprob * sizeof(double))
+    sub.alloc(size_t(n) * nprob * sizeof(double));
+  if (species.size() < size_t(n) * stride) species.alloc(size_t(n) * stride);
   cap = n;
-  cand.alloc(size_t(n) * sizeof(uint64_t));
-  obj.alloc(size_t(n) * sizeof(double));
-  sub.alloc(size_t(n) * nprob * sizeof(double));
-  species.alloc(size_t(n) * stride);
 }
 
 void RecordSink::bind(RunParams &p) const {
