@@ -109,9 +109,10 @@ __device__ __forceinline__ uint32_t tile_offset(uint32_t s, uint32_t I, uint32_t
 // (lane < npass * CW) and, when given, writes its ordered bonds and SRO cube.
 template <int TT, int G, int D, class PR>
 __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
-                                              uint32_t stride, uint32_t *cnt, uint32_t CW,
-                                              uint32_t npass, unsigned long long live, int lane,
-                                              int32_t *bonds, double *sro) {
+                                              uint32_t stride, uint32_t *cnt, double *vals,
+                                              uint32_t CW, uint32_t npass,
+                                              unsigned long long live, int lane, int32_t *bonds,
+                                              double *sro, uint32_t out_rows) {
   constexpr int P = TT * (TT + 1) / 2;
   const uint32_t K = pr.K, KK = K * K, T = pr.dT, S = pr.S;
   const uint32_t g = CW / G;
@@ -125,6 +126,13 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   const uint32_t toff = static_cast<uint32_t>(lane) * 16u;
   const uint32_t my_pass = static_cast<uint32_t>(lane) / CW;
   const uint32_t my_cc = static_cast<uint32_t>(lane) - my_pass * CW;  // candidate in its pass
+  // objective terms are evaluated lane-parallel: lane l takes terms idx = l, l + 64, ... of
+  // the pass (idx = candidate * Ts + term), stepped without division
+  const uint32_t Ts = K * (K + 1) / 2, nterm = CW * Ts;
+  const uint32_t step_t = 64u % Ts, step_c = 64u / Ts;
+  const uint32_t c_first = static_cast<uint32_t>(lane) / Ts;
+  const uint32_t t_first = static_cast<uint32_t>(lane) - c_first * Ts;
+  const uint32_t cube = pr.S * KK;
   uint32_t keep = 0;  // accumulator registers of GEMM2 on this lane's diagonal block
 #pragma unroll
   for (uint32_t r = 0; r < 16; ++r) {
@@ -214,28 +222,34 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
             cnt[gi * g * KK + K * m + b] = static_cast<uint32_t>(C[gi][r]);
           }
       wave_sync();
-      if (my_pass == pass) {
-        const uint32_t *cc = cnt + my_cc * KK;
-        auto count = [&](uint32_t a, uint32_t b2) { return cc[a * K + b2]; };
-        const uint32_t o = s * KK;
-        for (uint32_t xi = 0; xi < K; ++xi) {
-          const uint32_t cxx = count(xi, xi);
-          const double sxx = 1.0 - static_cast<double>(cxx) * ldc(pr.pref, o + xi * K + xi);
-          obj += ldc(pr.pw, o + xi * K + xi) * absolute(sxx - ldc(pr.tgt, o + xi * K + xi));
-          if (sro) sro[(s * K + xi) * K + xi] = sxx;
-          if (bonds) bonds[(s * K + xi) * K + xi] = static_cast<int32_t>(cxx);
-          for (uint32_t eta = xi + 1; eta < K; ++eta) {
-            const uint32_t cab = count(xi, eta), cba = count(eta, xi);
-            const uint32_t ix = o + xi * K + eta;
-            const double v = 1.0 - static_cast<double>(cab + cba) * ldc(pr.pref, ix);
-            obj += ldc(pr.pw, ix) * absolute(v - ldc(pr.tgt, ix));
-            if (sro) sro[(s * K + xi) * K + eta] = v, sro[(s * K + eta) * K + xi] = v;
-            if (bonds) {
-              bonds[(s * K + xi) * K + eta] = static_cast<int32_t>(cab);
-              bonds[(s * K + eta) * K + xi] = static_cast<int32_t>(cba);
-            }
+      // this shell's terms (optimization.h:85-100) for every candidate of the pass
+      {
+        const TermParam *tp = pr.tparams + s * Ts;
+        uint32_t c = c_first, t = t_first;
+        for (uint32_t idx = static_cast<uint32_t>(lane); idx < nterm; idx += 64) {
+          const TermParam e = tp[t];
+          const uint32_t *cc = cnt + c * KK;
+          const uint32_t cab = cc[e.xi * K + e.eta];
+          const uint32_t cba = e.xi == e.eta ? 0u : cc[e.eta * K + e.xi];
+          const double v = 1.0 - static_cast<double>(cab + cba) * e.pref;
+          vals[idx] = e.pw * absolute(v - e.tgt);
+          const uint32_t cr = pass * CW + c;  // candidate row
+          if ((bonds || sro) && cr < out_rows) {
+            const uint32_t o = cr * cube + (s * K + e.xi) * K + e.eta;
+            const uint32_t ot = cr * cube + (s * K + e.eta) * K + e.xi;
+            if (sro) sro[o] = v, sro[ot] = v;
+            if (bonds) bonds[o] = static_cast<int32_t>(cab), bonds[ot] = static_cast<int32_t>(cc[e.eta * K + e.xi]);
           }
+          t += step_t;
+          c += step_c;
+          if (t >= Ts) t -= Ts, ++c;
         }
+      }
+      wave_sync();
+      // each candidate's lane adds the shell's terms in the reference order
+      if (my_pass == pass) {
+        const double *vv = vals + my_cc * Ts;
+        for (uint32_t t = 0; t < Ts; ++t) obj += vv[t];
       }
       wave_sync();
     }
@@ -261,6 +275,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
   const uint32_t npass = GEN / CW;
   uint8_t *rows = wl + p.off_rows[0];
   uint32_t *cnt = reinterpret_cast<uint32_t *>(wl + p.off_hist);
+  double *vals = reinterpret_cast<double *>(wl + p.off_obj);
   const bool gl = static_cast<uint32_t>(lane) < GEN;
   uint8_t *row = rows + (gl ? static_cast<uint32_t>(lane) : 0u) * stride;
   for (uint32_t r = 0; r < p.rounds; ++r) {
@@ -288,7 +303,8 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
       if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
     wave_sync();
     const double tot =
-        dense_score<TT, G, D>(pr, rows, stride, cnt, CW, npass, live, lane, nullptr, nullptr);
+        dense_score<TT, G, D>(pr, rows, stride, cnt, vals, CW, npass, live, lane, nullptr, nullptr,
+                              0u);
     scored += static_cast<unsigned long long>(__popcll(am));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -370,6 +386,7 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   const uint32_t GEN = p.dense_gen, CW = p.dense_cw, npass = GEN / CW;
   uint8_t *rows = lds + p.off_row;
   uint32_t *cnt = reinterpret_cast<uint32_t *>(lds + p.off_hist);
+  double *vals = reinterpret_cast<double *>(lds + p.off_terms);
   const uint32_t nb = pr.S * pr.K * pr.K;
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * GEN + lane;
   const bool gl = static_cast<uint32_t>(lane) < GEN;
@@ -384,9 +401,11 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   for (uint32_t q = 0; q < npass; ++q)
     if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
   wave_sync();
-  const double obj = dense_score<TT, G, D>(pr, rows, p.dense_stride, cnt, CW, npass, live, lane,
-                                        active && p.bonds ? p.bonds + b * nb : nullptr,
-                                        active && p.sro ? p.sro + b * nb : nullptr);
+  const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * GEN;
+  const uint32_t out_rows = static_cast<uint32_t>(p.B - b0 < GEN ? p.B - b0 : GEN);
+  const double obj = dense_score<TT, G, D>(pr, rows, p.dense_stride, cnt, vals, CW, npass, live,
+                                           lane, p.bonds ? p.bonds + b0 * nb : nullptr,
+                                           p.sro ? p.sro + b0 * nb : nullptr, out_rows);
   if (active) p.objective[b] = obj;
 }
 

@@ -12,6 +12,14 @@ constexpr int kMaxLocalSpecies = 16;
 constexpr uint32_t kPairIndexBits = 12;  // packed pair: i | j << 12 | shell << 24
 constexpr uint32_t kMaxSites = 1u << kPairIndexBits;
 
+// One objective term (optimization.h:85-100) in reference order: shell s, then xi, then
+// eta = xi (the diagonal) and eta > xi; parameters of (s, xi, eta), pw scaled by the shell
+// weight.
+struct TermParam {
+  double pref, pw, tgt;
+  uint32_t xi, eta;
+};
+
 // One sublattice problem as the kernels see it (optimization_config_data, device side).
 struct DevProblem {
   const uint32_t *pairs;      // P packed pairs, i < j in sorted-site order
@@ -20,6 +28,7 @@ struct DevProblem {
   const double *pw;           // [S][K][K] scaled by the shell weights
   const double *tgt;          // [S][K][K]
   const uint32_t *terms;      // T objective terms in reference order: s | xi << 8 | eta << 16
+  const TermParam *tparams;   // the same T terms with their parameters
   // dense contraction: 32 x 32 int8 tiles of the upper shell adjacency in the A-operand
   // layout of v_mfma_i32_32x32x32_i8, every tile I <= J < dT of every shell, in (shell,
   // row tile, column tile) order

@@ -176,6 +176,13 @@ void Problem::upload_device() {
   upload(d_pw, pw);
   upload(d_tgt, tgt);
   upload(d_terms, terms);
+  std::vector<TermParam> tp(terms.size());
+  for (size_t t = 0; t < terms.size(); ++t) {
+    const uint32_t s = terms[t] & 0xff, xi = (terms[t] >> 8) & 0xff, eta = terms[t] >> 16;
+    const size_t ix = (size_t(s) * K + xi) * K + eta;
+    tp[t] = TermParam{pref[ix], pw[ix], tgt[ix], xi, eta};
+  }
+  upload(d_tparams, tp);
 }
 
 void Problem::build_lane(uint32_t field_bits) {
@@ -302,6 +309,7 @@ DevProblem Problem::dev() const {
   d.pw = d_pw.as<const double>();
   d.tgt = d_tgt.as<const double>();
   d.terms = d_terms.as<const uint32_t>();
+  d.tparams = d_tparams.as<const TermParam>();
   d.lane_stream = d_lane_stream.as<const uint32_t>();
   d.lane_off = d_lane_off.as<const uint32_t>();
   d.species0F = d_species0F.as<const uint8_t>();
@@ -505,20 +513,26 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   uint32_t off = 0;
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
-    // rows of gen_group candidates (npass passes of dense_cw) + one shell's counts
+    // rows of gen_group candidates (npass passes of dense_cw), then one shell's counts and
+    // term values
     const Problem &P = job->problems[0];
     const uint32_t T = (P.N + 31) / 32, G = dense_groups(dense_tile_bound(T));
     const uint32_t g = std::min(32 / P.K, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
-    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4);  // one shell's counts
+    // one shell's counts and term values of a pass
+    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4);
+    const uint32_t val_bytes = align(p.dense_cw * (P.K * (P.K + 1) / 2) * 8);
     uint32_t npass = 64 / p.dense_cw;
     // at most 20 KiB per wavefront: eight resident wavefronts per CU (two per SIMD)
-    while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 20 * 1024) --npass;
+    while (npass > 1 &&
+           npass * p.dense_cw * p.dense_stride + cnt_bytes + val_bytes > 20 * 1024)
+      --npass;
     p.gen_group = npass * p.dense_cw;
     p.off_rows[0] = 0;
     p.off_hist = align(p.gen_group * p.dense_stride);
-    p.wave_lds = p.off_hist + cnt_bytes;
+    p.off_obj = p.off_hist + cnt_bytes;
+    p.wave_lds = p.off_obj + val_bytes;
     // one wavefront per block, as many as registers and LDS keep resident
     *blocks = static_cast<uint32_t>(device_cu_count()) *
               dense_waves_per_cu(dense_tile_bound(T), p.wave_lds, false);
@@ -740,6 +754,7 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     sp.dense_stride = rp.dense_stride;
     sp.off_row = rp.off_rows[0];
     sp.off_hist = rp.off_hist;
+    sp.off_terms = rp.off_obj;
     sp.wave_lds = rp.wave_lds;
     SQSG_HIP(launch_score_dense(sp, dense_tile_bound_of(job), s ? s : job->stream));
     return;

@@ -73,7 +73,7 @@ struct Problem {
   uint32_t sys_hist[kMaxLocalSpecies] = {};
   uint8_t sys_rev[kMaxLocalSpecies] = {};
   // device copies
-  DevBuf d_pairs, d_species0, d_pref, d_pw, d_tgt, d_terms;
+  DevBuf d_pairs, d_species0, d_pref, d_pw, d_tgt, d_terms, d_tparams;
   // lane-per-candidate layout (built for the job's field width F)
   std::vector<uint32_t> lane_stream, lane_off;
   uint32_t F = 0;
