@@ -61,6 +61,8 @@ extern "C" {
 #define SQSG_KERNEL_SPARSE_WAVE 1   /* one wavefront per candidate, LDS pair histogram */
 #define SQSG_KERNEL_SPARSE_LANE 2   /* one lane per candidate, LDS [bin][lane] counters */
 #define SQSG_KERNEL_DENSE_MFMA 3    /* int8 one-hot x int8 shell adjacency on MFMA, int32 accumulate */
+#define SQSG_KERNEL_DENSE_PAIRED 4  /* the dense contraction with an MFMA and a helper wavefront
+                                       paired on each SIMD (Monte Carlo jobs, <= 256 sites) */
 
 typedef struct sqsg_job sqsg_job;
 typedef struct sqsg_server sqsg_server;

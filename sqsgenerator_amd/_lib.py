@@ -18,7 +18,8 @@ ERR_DEVICE, ERR_NO_MEMORY, ERR_CANCELLED = 5, 6, 7
 
 ITERATION_RANDOM, ITERATION_SYSTEMATIC = 0, 1
 SUBLATTICE_INTERACT, SUBLATTICE_SPLIT = 0, 1
-KERNEL_AUTO, KERNEL_SPARSE_WAVE, KERNEL_SPARSE_LANE, KERNEL_DENSE_MFMA = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_SPARSE_WAVE, KERNEL_SPARSE_LANE, KERNEL_DENSE_MFMA, KERNEL_DENSE_PAIRED = \
+    0, 1, 2, 3, 4
 
 u8p = C.POINTER(C.c_uint8)
 u32p = C.POINTER(C.c_uint32)

@@ -262,7 +262,7 @@ int sqsg_score_batch_device(sqsg_job *job, uint32_t p, const uint8_t *d_species,
 int sqsg_job_set_kernel(sqsg_job *job, int kernel) {
   return guard([&] {
     need(job, "job");
-    if (kernel < SQSG_KERNEL_AUTO || kernel > SQSG_KERNEL_DENSE_MFMA)
+    if (kernel < SQSG_KERNEL_AUTO || kernel > SQSG_KERNEL_DENSE_PAIRED)
       throw std::invalid_argument("unknown kernel variant");
     job->kernel = kernel;
   });

@@ -84,7 +84,8 @@ struct Slot {
 
 struct Group {
   int variant = 0;
-  uint32_t key = 0;  // lane: largest species count; dense: tile bound
+  uint32_t key = 0;    // the group's identity within its variant
+  uint32_t shape = 0;  // launch template: lane: largest species count; dense: tile bound
   DevBuf d_jobs, d_items, d_counts, d_claims;
   hipGraphExec_t exec = nullptr;
   uint32_t blocks = 0;     // of the current step's launch
@@ -160,7 +161,8 @@ class BatchEngine {
     if (variant == SQSG_KERNEL_SPARSE_WAVE)
       throw std::invalid_argument("batched execution needs a job the lane or dense kernel "
                                   "handles");
-    if (variant == SQSG_KERNEL_DENSE_MFMA) ensure_dense(job);
+    const bool dense = variant == SQSG_KERNEL_DENSE_MFMA || variant == SQSG_KERNEL_DENSE_PAIRED;
+    if (dense) ensure_dense(job);
     auto s = std::make_unique<Slot>();
     s->job = job;
     uint32_t blocks = 0;
@@ -175,7 +177,8 @@ class BatchEngine {
     } else {
       s->p.rounds = 1;
     }
-    s->chunk = gen * s->p.rounds;
+    // a paired block claims one batch for its four pairs at a time
+    s->chunk = gen * s->p.rounds * (variant == SQSG_KERNEL_DENSE_PAIRED ? kDensePairs : 1);
     s->slice = std::max(s->slice, s->chunk);
     s->p.rec_stride = job->stride;
     if (!pool_.empty()) {
@@ -196,8 +199,13 @@ class BatchEngine {
                             hipMemcpyHostToDevice, stream_));
     s->p.thr_key = s->buf->thr.as<unsigned long long>();
     s->footprint = footprint(*s);
-    Group &g = group(variant, variant == SQSG_KERNEL_DENSE_MFMA ? dense_tile_bound_of(job)
-                                                                : job->kmax);
+    // groups share a launch shape: the tile bound (dense), also K (paired: one LDS layout),
+    // the largest species count (lane)
+    const uint32_t key = !dense ? job->kmax
+                         : variant == SQSG_KERNEL_DENSE_PAIRED
+                             ? dense_tile_bound_of(job) | (job->problems[0].K << 8)
+                             : dense_tile_bound_of(job);
+    Group &g = group(variant, key, dense ? dense_tile_bound_of(job) : job->kmax);
     if (g.members.size() >= kGroupJobs) throw std::out_of_range("too many resident jobs");
     g.members.push_back(s.get());
     s->group = &g;
@@ -333,12 +341,13 @@ class BatchEngine {
     }
   }
 
-  Group &group(int variant, uint32_t key) {
-    auto &g = groups_[(static_cast<uint32_t>(variant) << 8) | key];
+  Group &group(int variant, uint32_t key, uint32_t shape) {
+    auto &g = groups_[(static_cast<uint64_t>(variant) << 32) | key];
     if (!g) {
       g = std::make_unique<Group>();
       g->variant = variant;
       g->key = key;
+      g->shape = shape;
       g->d_jobs.alloc(kGroupJobs * sizeof(RunParams));
       g->d_items.alloc(kGroupItems * sizeof(WorkItem));
       g->d_counts.alloc(2 * sizeof(uint32_t));
@@ -385,10 +394,12 @@ class BatchEngine {
     SQSG_HIP(hipMemcpyAsync(g.d_counts.as<void>(), g.h_counts[q], 2 * sizeof(uint32_t),
                             hipMemcpyHostToDevice, stream_));
     const size_t block_lds = wave_lds;
-    const bool dense = g.variant == SQSG_KERNEL_DENSE_MFMA;
     uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / std::max<size_t>(block_lds, 1));
     per_cu = std::max(1u, std::min(per_cu, 16u));
-    if (dense) per_cu = dense_waves_per_cu(g.key, block_lds, true);
+    if (g.variant == SQSG_KERNEL_DENSE_MFMA)
+      per_cu = dense_waves_per_cu(g.shape, block_lds, true);
+    else if (g.variant == SQSG_KERNEL_DENSE_PAIRED)
+      per_cu = dense_paired_blocks_per_cu(g.shape, block_lds, true);
     const uint32_t blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
     BatchParams &bp = g.bp;
     bp = BatchParams{};
@@ -400,7 +411,9 @@ class BatchEngine {
     bp.claims = g.d_claims.as<unsigned int>();
     bp.max_items = kGroupItems;
     g.blocks = blocks;
-    SQSG_HIP(dense ? prepare_batch_dense(g.key, block_lds) : prepare_batch_lane(g.key, block_lds));
+    SQSG_HIP(g.variant == SQSG_KERNEL_DENSE_MFMA     ? prepare_batch_dense(g.shape, block_lds)
+             : g.variant == SQSG_KERNEL_DENSE_PAIRED ? prepare_batch_dense_paired(g.shape, block_lds)
+                                                     : prepare_batch_lane(g.shape, block_lds));
     if (use_graph_ && (!g.exec || g.graph_wave_lds != wave_lds || g.graph_blocks != blocks)) {
       // the graph covers every table row (num_jobs is a kernel argument of the reset)
       if (g.exec) SQSG_HIP(hipGraphExecDestroy(g.exec));
@@ -422,9 +435,12 @@ class BatchEngine {
   }
 
   hipError_t launch_kernel(const Group &g) {
-    return g.variant == SQSG_KERNEL_DENSE_MFMA
-               ? launch_batch_dense(g.bp, g.key, g.blocks, stream_)
-               : launch_batch_lane(g.bp, g.key, g.blocks, 1, stream_);
+    switch (g.variant) {
+      case SQSG_KERNEL_DENSE_MFMA: return launch_batch_dense(g.bp, g.shape, g.blocks, stream_);
+      case SQSG_KERNEL_DENSE_PAIRED:
+        return launch_batch_dense_paired(g.bp, g.shape, g.blocks, stream_);
+      default: return launch_batch_lane(g.bp, g.shape, g.blocks, 1, stream_);
+    }
   }
 
   // the step's launches only (tables, attributes and graphs are prepared beforehand)
@@ -447,7 +463,7 @@ class BatchEngine {
   std::vector<std::unique_ptr<Slot>> slots_;
   std::vector<std::unique_ptr<Slot>> retired_[2];
   std::vector<std::unique_ptr<SlotBuffers>> pool_;
-  std::map<uint32_t, std::unique_ptr<Group>> groups_;
+  std::map<uint64_t, std::unique_ptr<Group>> groups_;
 };
 
 // One engine per graph mode for the whole process: its groups, device tables and captured
@@ -489,6 +505,7 @@ void autotune(sqsg_job *job, int *chosen, double *rate) {
   std::vector<int> variants{SQSG_KERNEL_SPARSE_WAVE};
   if (job->lane_ok) variants.push_back(SQSG_KERNEL_SPARSE_LANE);
   if (job->dense_ok) variants.push_back(SQSG_KERNEL_DENSE_MFMA);
+  if (paired_ok(job)) variants.push_back(SQSG_KERNEL_DENSE_PAIRED);
   const uint64_t saved_it = job->iterations, saved_launch = job->launch_size;
   const int saved_kernel = job->kernel;
   uint64_t work = 0;

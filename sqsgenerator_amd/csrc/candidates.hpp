@@ -24,17 +24,31 @@ template <uint32_t SH> struct Strided {
 
 // PR is DevProblem, possibly qualified with the constant address space (a job table entry
 // read with scalar loads)
+__device__ __forceinline__ uint32_t bounded(uint32_t r, uint32_t n) {  // shuffle.h:28-32
+  return static_cast<uint32_t>((static_cast<uint64_t>(r) * n) >> 32);
+}
+
+// The draw of step i - 1 is computed between the reads and the writes of step i, so its
+// multiplies cover the LDS round trip of the swap (the byte reads and writes of one lane
+// stay in program order, and the draws are consumed in the same order as a plain loop).
 template <uint32_t SH, class PR>
 __device__ __forceinline__ void shuffle_candidate(Strided<SH> v, const PR &pr, uint64_t cand) {
   uint64_t st = pr.seed + (cand * static_cast<uint64_t>(pr.draws)) * kSecret0;
   for (uint32_t b = 0; b < pr.nbounds; ++b) {
     const uint32_t lo = pr.bounds[2 * b], hi = pr.bounds[2 * b + 1];
-    for (uint32_t i = hi - lo; i > 1; --i) {
-      const uint32_t p = static_cast<uint32_t>((static_cast<uint64_t>(draw32(st)) * i) >> 32);
-      const uint8_t t = v[lo + i - 1];
-      v[lo + i - 1] = v[lo + p];
-      v[lo + p] = t;
+    uint32_t i = hi - lo;
+    if (i < 2) continue;
+    uint32_t p = bounded(draw32(st), i);
+    for (; i > 2; --i) {
+      const uint8_t a = v[lo + i - 1], c = v[lo + p];
+      const uint32_t pn = bounded(draw32(st), i - 1);
+      v[lo + i - 1] = c;
+      v[lo + p] = a;
+      p = pn;
     }
+    const uint8_t a = v[lo + 1], c = v[lo + p];  // i == 2
+    v[lo + 1] = c;
+    v[lo + p] = a;
   }
 }
 

@@ -60,9 +60,6 @@ __device__ __forceinline__ unsigned long long low_mask(uint32_t n) {
 
 // read-only problem data through the constant address space: uniform loads become scalar
 // loads, which do not wait behind the tile stream's vector loads
-template <class T> __device__ __forceinline__ T ldc(const T *p, uint32_t i) {
-  return ((const __attribute__((address_space(4))) T *)p)[i];
-}
 template <class T>
 __device__ __forceinline__ const __attribute__((address_space(4))) T &constant_ref(const T *p,
                                                                                    uint32_t i) {
@@ -86,7 +83,7 @@ template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
     case 1: return f(DenseShape<1, 4, 2, 1>{});
     case 2: return f(DenseShape<2, 4, 2, 3>{});
     case 4: return f(DenseShape<4, 2, 2, 5>{});
-    case 8: return f(DenseShape<8, 2, 2, 12>{});
+    case 8: return f(DenseShape<8, 2, 2, 9>{});
     case 16: return f(DenseShape<16, 2, 1, 8>{});
     case 28: return f(DenseShape<28, 1, 1, 14>{});
     case 32: return f(DenseShape<32, 1, 1, 16>{});
@@ -103,13 +100,56 @@ __device__ __forceinline__ uint32_t tile_offset(uint32_t s, uint32_t I, uint32_t
   return (s * per_shell + I * T - I * (I - 1) / 2 + (J - I)) * 1024u;
 }
 
+// One shell's objective terms (optimization.h:85-100) of the candidate whose K x K counts
+// are at cc (LDS), added to obj in the reference order (xi, then eta >= xi); bonds and SRO
+// cubes [S][K][K] are written when given.  The term parameters are uniform across the
+// wavefront and come through scalar loads; with K known at compile time every count read
+// has a constant offset and all loads are issued ahead of the arithmetic.
+template <int KT, class PR>
+__device__ __forceinline__ void shell_terms_k(const PR &pr, uint32_t s, const uint32_t *cc,
+                                              double &obj, int32_t *bonds, double *sro,
+                                              uint32_t Kr) {
+  const uint32_t K = KT > 0 ? static_cast<uint32_t>(KT) : Kr;
+  const uint32_t Ts = K * (K + 1) / 2;
+  const TermParam *tp = pr.tparams + s * Ts;
+  uint32_t t = 0;
+#pragma unroll
+  for (uint32_t xi = 0; xi < K; ++xi)
+#pragma unroll
+    for (uint32_t eta = xi; eta < K; ++eta, ++t) {
+      const auto &e = constant_ref(tp, t);
+      const uint32_t cab = cc[xi * K + eta];
+      const uint32_t cba = xi == eta ? 0u : cc[eta * K + xi];
+      const double v = 1.0 - static_cast<double>(cab + cba) * e.pref;
+      obj += e.pw * absolute(v - e.tgt);
+      if (sro) sro[(s * K + xi) * K + eta] = v, sro[(s * K + eta) * K + xi] = v;
+      if (bonds) {
+        bonds[(s * K + xi) * K + eta] = static_cast<int32_t>(cab);
+        bonds[(s * K + eta) * K + xi] = static_cast<int32_t>(cc[eta * K + xi]);
+      }
+    }
+}
+
+template <class PR>
+__device__ __forceinline__ void shell_terms(const PR &pr, uint32_t s, const uint32_t *cc,
+                                            double &obj, int32_t *bonds, double *sro) {
+  switch (pr.K) {
+    case 2: return shell_terms_k<2>(pr, s, cc, obj, bonds, sro, 2);
+    case 3: return shell_terms_k<3>(pr, s, cc, obj, bonds, sro, 3);
+    case 4: return shell_terms_k<4>(pr, s, cc, obj, bonds, sro, 4);
+    case 5: return shell_terms_k<5>(pr, s, cc, obj, bonds, sro, 5);
+    case 6: return shell_terms_k<6>(pr, s, cc, obj, bonds, sro, 6);
+    default: return shell_terms_k<0>(pr, s, cc, obj, bonds, sro, pr.K);
+  }
+}
+
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
 // bytes, padding sites 0xff); `cnt` is CW * K * K words of LDS.  Passes whose bit in `live`
 // is clear are skipped.  Returns the objective of the lane's own candidate
 // (lane < npass * CW) and, when given, writes its ordered bonds and SRO cube.
 template <int TT, int G, int D, class PR>
 __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
-                                              uint32_t stride, uint32_t *cnt, double *vals,
+                                              uint32_t stride, uint32_t *cnt,
                                               uint32_t CW, uint32_t npass,
                                               unsigned long long live, int lane, int32_t *bonds,
                                               double *sro, uint32_t out_rows) {
@@ -126,12 +166,6 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   const uint32_t toff = static_cast<uint32_t>(lane) * 16u;
   const uint32_t my_pass = static_cast<uint32_t>(lane) / CW;
   const uint32_t my_cc = static_cast<uint32_t>(lane) - my_pass * CW;  // candidate in its pass
-  // objective terms are evaluated lane-parallel: lane l takes terms idx = l, l + 64, ... of
-  // the pass (idx = candidate * Ts + term), stepped without division
-  const uint32_t Ts = K * (K + 1) / 2, nterm = CW * Ts;
-  const uint32_t step_t = 64u % Ts, step_c = 64u / Ts;
-  const uint32_t c_first = static_cast<uint32_t>(lane) / Ts;
-  const uint32_t t_first = static_cast<uint32_t>(lane) - c_first * Ts;
   const uint32_t cube = pr.S * KK;
   uint32_t keep = 0;  // accumulator registers of GEMM2 on this lane's diagonal block
 #pragma unroll
@@ -222,36 +256,15 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
             cnt[gi * g * KK + K * m + b] = static_cast<uint32_t>(C[gi][r]);
           }
       wave_sync();
-      // this shell's terms (optimization.h:85-100) for every candidate of the pass
-      {
-        const TermParam *tp = pr.tparams + s * Ts;
-        uint32_t c = c_first, t = t_first;
-        for (uint32_t idx = static_cast<uint32_t>(lane); idx < nterm; idx += 64) {
-          const TermParam e = tp[t];
-          const uint32_t *cc = cnt + c * KK;
-          const uint32_t cab = cc[e.xi * K + e.eta];
-          const uint32_t cba = e.xi == e.eta ? 0u : cc[e.eta * K + e.xi];
-          const double v = 1.0 - static_cast<double>(cab + cba) * e.pref;
-          vals[idx] = e.pw * absolute(v - e.tgt);
-          const uint32_t cr = pass * CW + c;  // candidate row
-          if ((bonds || sro) && cr < out_rows) {
-            const uint32_t o = cr * cube + (s * K + e.xi) * K + e.eta;
-            const uint32_t ot = cr * cube + (s * K + e.eta) * K + e.xi;
-            if (sro) sro[o] = v, sro[ot] = v;
-            if (bonds) bonds[o] = static_cast<int32_t>(cab), bonds[ot] = static_cast<int32_t>(cc[e.eta * K + e.xi]);
-          }
-          t += step_t;
-          c += step_c;
-          if (t >= Ts) t -= Ts, ++c;
-        }
-      }
-      wave_sync();
-      // each candidate's lane adds the shell's terms in the reference order
+      // this shell's terms (optimization.h:85-100), added by each candidate's own lane in
+      // the reference order; the term parameters are uniform (scalar loads)
       if (my_pass == pass) {
-        const double *vv = vals + my_cc * Ts;
-        for (uint32_t t = 0; t < Ts; ++t) obj += vv[t];
+        const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
+        const bool out = (bonds || sro) && cr < out_rows;
+        shell_terms(pr, s, cnt + my_cc * KK, obj, out ? bonds + cr * cube : nullptr,
+                    out ? sro + cr * cube : nullptr);
       }
-      wave_sync();
+      wave_sync();  // the counts are rewritten by the next shell
     }
   }
   return obj;
@@ -275,7 +288,6 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
   const uint32_t npass = GEN / CW;
   uint8_t *rows = wl + p.off_rows[0];
   uint32_t *cnt = reinterpret_cast<uint32_t *>(wl + p.off_hist);
-  double *vals = reinterpret_cast<double *>(wl + p.off_obj);
   const bool gl = static_cast<uint32_t>(lane) < GEN;
   uint8_t *row = rows + (gl ? static_cast<uint32_t>(lane) : 0u) * stride;
   for (uint32_t r = 0; r < p.rounds; ++r) {
@@ -303,7 +315,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
       if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
     wave_sync();
     const double tot =
-        dense_score<TT, G, D>(pr, rows, stride, cnt, vals, CW, npass, live, lane, nullptr, nullptr,
+        dense_score<TT, G, D>(pr, rows, stride, cnt, CW, npass, live, lane, nullptr, nullptr,
                               0u);
     scored += static_cast<unsigned long long>(__popcll(am));
     const unsigned long long g =
@@ -386,7 +398,6 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   const uint32_t GEN = p.dense_gen, CW = p.dense_cw, npass = GEN / CW;
   uint8_t *rows = lds + p.off_row;
   uint32_t *cnt = reinterpret_cast<uint32_t *>(lds + p.off_hist);
-  double *vals = reinterpret_cast<double *>(lds + p.off_terms);
   const uint32_t nb = pr.S * pr.K * pr.K;
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * GEN + lane;
   const bool gl = static_cast<uint32_t>(lane) < GEN;
@@ -403,10 +414,375 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   wave_sync();
   const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * GEN;
   const uint32_t out_rows = static_cast<uint32_t>(p.B - b0 < GEN ? p.B - b0 : GEN);
-  const double obj = dense_score<TT, G, D>(pr, rows, p.dense_stride, cnt, vals, CW, npass, live,
+  const double obj = dense_score<TT, G, D>(pr, rows, p.dense_stride, cnt, CW, npass, live,
                                            lane, p.bonds ? p.bonds + b0 * nb : nullptr,
                                            p.sro ? p.sro + b0 * nb : nullptr, out_rows);
   if (active) p.objective[b] = obj;
+}
+
+// ---- paired form (MC jobs) -------------------------------------------------------------
+//
+// A block is one CU's worth of wavefronts: four pairs of an MFMA wavefront (M, waves 0-3)
+// and a helper (H, waves 4-7; wave w + 4 shares the SIMD of wave w).  All eight advance
+// on a common tick, one tick per (pass, shell) of a batch of gen_group candidates per pair:
+//   M at a tick of batch b: the one-hot tiles (first shell of a pass), the shell's GEMMs
+//     and its diagonal-block counts into cnt[tick parity];
+//   H at the same tick: the objective terms of the previous tick's counts (summed per
+//     candidate in reference order); at a batch's first tick the end of batch b - 1 (tie
+//     filter, records); at every tick a slice of the Fisher-Yates shuffles of batch b + 1.
+// Candidate rows are double-buffered by batch parity, counts by tick parity, and one
+// __syncthreads per tick separates producers and consumers.  Wave 4 claims the block's
+// batches two ahead into three LDS descriptors; every wave reads the same descriptors, so
+// all run the same ticks and barriers.  Without generation and objective code the MFMA
+// wavefront keeps its registers for the one-hot tiles, the tile ring and the accumulators,
+// and the helper's vector and LDS work runs beside the matrix core on the same SIMD.
+//
+// LDS: descriptors in [0, 128), then pair p's region at 128 + p * ws_pair: rows
+// off_rows[0] / off_rows[1], counts off_hist / off_terms.
+
+constexpr uint32_t kWsPairs = kDensePairs, kWsHead = kDensePairHead;
+
+struct WsBatch {  // a block batch: candidates [cbeg, cend) of job `job`
+  uint64_t cbeg, cend;
+  uint32_t job, valid;
+};
+
+// claim sources: the single-job kernel takes block batches of one job from a counter, the
+// persistent kernel one-batch chunks of a work list
+struct RunSource {
+  const RunParams &p;
+  __device__ WsBatch next(int lane) {
+    const uint64_t per = static_cast<uint64_t>(kWsPairs) * p.gen_group;
+    const uint64_t nseg = (p.cand_end - p.cand_begin + per - 1) / per;
+    uint32_t seg = 0;
+    if (lane == 0) seg = atomicAdd(p.next_seg, 1u);
+    seg = __builtin_amdgcn_readfirstlane(__shfl(seg, 0));
+    if (seg >= nseg) return WsBatch{0, 0, 0, 0};
+    const uint64_t b = p.cand_begin + seg * per;
+    return WsBatch{b, b + per < p.cand_end ? b + per : p.cand_end, 0, 1};
+  }
+  __device__ const RunParams &params(uint32_t) const { return p; }
+};
+
+struct BatchSource {
+  const BatchParams &bp;
+  uint32_t num_items, it;
+  __device__ WsBatch next(int lane) {
+    uint32_t job = 0;
+    uint64_t cb = 0, ce = 0;
+    if (!claim_chunk(bp, num_items, it, lane, job, cb, ce)) return WsBatch{0, 0, 0, 0};
+    return WsBatch{cb, ce, job, 1};
+  }
+  __device__ const __attribute__((address_space(4))) RunParams &params(uint32_t job) const {
+    return constant_ref(bp.jobs, job);
+  }
+};
+
+// M: one (pass, shell).  X and the ring live across the ticks of a pass: the ring
+// prefetches the next shell's first positions (zeros past the last shell) and is refilled
+// with the X tiles at the first shell of a pass.
+template <int TT, int G, int D, class PR>
+__device__ __forceinline__ void ws_mfma_tick(const PR &pr, const uint8_t *rows, uint32_t stride,
+                                             uint32_t CW, uint32_t pass, uint32_t s,
+                                             v4i (&X)[G][TT], v4i (&ring)[D], uint32_t *cnt,
+                                             int lane) {
+  constexpr int P = TT * (TT + 1) / 2;
+  const uint32_t K = pr.K, KK = K * K, T = pr.dT, S = pr.S;
+  const uint32_t g = CW / G;
+  const uint32_t h = static_cast<uint32_t>(lane) >> 5, n = static_cast<uint32_t>(lane) & 31u;
+  const uint32_t cl = n / K, b = n - cl * K;
+  const bool col_ok = cl < g;
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t *>(pr.dtiles), static_cast<short>(0),
+      static_cast<int>(pr.dtiles_n * 1024u), 0x00020000);
+  const uint32_t toff = static_cast<uint32_t>(lane) * 16u;
+  if (s == 0) {
+    const uint32_t bb = b * 0x01010101u;
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      const uint32_t c = pass * CW + gi * g + cl;
+      const uint8_t *row = rows + (col_ok ? c : 0u) * stride + 16 * h;
+#pragma unroll
+      for (int J = 0; J < TT; ++J) {
+        v4i x = {0, 0, 0, 0};
+        if (static_cast<uint32_t>(J) < T && col_ok) {
+          const uint4 w = *reinterpret_cast<const uint4 *>(row + 32 * J);
+          x[0] = onehot4(w.x, bb);
+          x[1] = onehot4(w.y, bb);
+          x[2] = onehot4(w.z, bb);
+          x[3] = onehot4(w.w, bb);
+        }
+        X[gi][J] = x;
+      }
+    }
+    int I2 = 0, J2 = 0;
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      ring[q] = __builtin_bit_cast(
+          v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, tile_offset(0, I2, J2, S, T) + toff,
+                                                     0, 0));
+      if (++J2 == TT) ++I2, J2 = I2;
+    }
+  }
+  v16i C[G], Y[G];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) C[gi] = v16i{};
+  int I = 0, J = 0, I2 = 0, J2 = 0, s2 = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    if (++J2 == TT) {
+      ++I2, J2 = I2;
+      if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
+    }
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    if (J == I) {
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
+    }
+    const v4i u = ring[q % D];
+    ring[q % D] = __builtin_bit_cast(
+        v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, tile_offset(s + s2, I2, J2, S, T) + toff,
+                                                   0, 0));
+    if (++J2 == TT) {
+      ++I2, J2 = I2;
+      if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
+    }
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+      Y[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(u, X[gi][J], Y[gi], 0, 0, 0);
+    if (J == TT - 1) {
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi)
+        C[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(X[gi][I], pack_counts(Y[gi]), C[gi], 0,
+                                                      0, 0);
+    }
+    if (++J == TT) ++I, J = I;
+  }
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+    for (uint32_t r = 0; r < 16; ++r) {
+      const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
+      if (col_ok && m / K == cl) cnt[gi * g * KK + K * m + b] = static_cast<uint32_t>(C[gi][r]);
+    }
+}
+
+// H: the objective terms of one (pass, shell) from its counts, added to the running
+// objectives of the pass's candidates (the lane of candidate row c holds its sum)
+template <class PR>
+__device__ __forceinline__ void ws_objective(const PR &pr, uint32_t CW, uint32_t pass,
+                                             uint32_t s, const uint32_t *cnt, double &obj,
+                                             int lane) {
+  const uint32_t c = static_cast<uint32_t>(lane) - pass * CW;
+  if (c < CW) shell_terms(pr, s, cnt + c * pr.K * pr.K, obj, nullptr, nullptr);
+}
+
+// H: Fisher-Yates state of the lane's candidate (the steps of shuffle_candidate spread
+// over ticks): window w, its low bound and the current i
+struct FyState {
+  uint64_t st;
+  uint32_t w, lo, i;
+};
+
+template <class PR>
+__device__ __forceinline__ void fy_init(FyState &f, const PR &pr, uint64_t cand, bool shuffle) {
+  f.st = pr.seed + (cand * static_cast<uint64_t>(pr.draws)) * kSecret0;
+  f.w = shuffle ? 0u : pr.nbounds;
+  f.lo = pr.bounds[0];
+  f.i = pr.bounds[1] - pr.bounds[0];
+}
+
+// at most `n` swaps
+template <class PR>
+__device__ __forceinline__ void fy_steps(FyState &f, const PR &pr, uint8_t *row, uint32_t n) {
+  while (n > 0 && f.w < pr.nbounds) {
+    if (f.i > 1) {
+      const uint32_t q = static_cast<uint32_t>((static_cast<uint64_t>(draw32(f.st)) * f.i) >> 32);
+      const uint8_t tmp = row[f.lo + f.i - 1];
+      row[f.lo + f.i - 1] = row[f.lo + q];
+      row[f.lo + q] = tmp;
+      --f.i;
+      --n;
+    } else if (++f.w < pr.nbounds) {
+      f.lo = pr.bounds[2 * f.w];
+      f.i = pr.bounds[2 * f.w + 1] - f.lo;
+    }
+  }
+}
+
+// H: tie filter and records of a pair's finished batch [cbeg, cend) (as in dense_segment)
+template <class RP>
+__device__ __forceinline__ void ws_record(const RP &p, const uint8_t *rows, uint64_t cbeg,
+                                          uint64_t cend, double tot, LevelFilter &lf,
+                                          int lane) {
+  const auto &pr = p.prob[0];
+  const uint64_t cand = cbeg + static_cast<uint32_t>(lane);
+  const bool active = static_cast<uint32_t>(lane) < p.gen_group && cand < cend;
+  const unsigned long long am = __ballot(active);
+  if (am == 0) return;
+  if (lane == 0) atomicAdd(p.scored, static_cast<unsigned long long>(__popcll(am)));
+  const unsigned long long g =
+      __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const bool below = active && okey(tot) <= g;
+  offer_levels(lf, p, __ballot(below), tot, g, lane);
+  const unsigned long long lk = lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
+  if (below && okey(tot) <= lk) {
+    const uint32_t slot = atomicAdd(p.rec_count, 1u);
+    if (slot < p.rec_cap) {
+      const uint8_t *row = rows + static_cast<uint32_t>(lane) * p.dense_stride;
+      uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
+      for (uint32_t i = 0; i < pr.N; ++i) dst[i] = row[i];
+      p.rec_cand[slot] = cand;
+      p.rec_obj[slot] = tot;
+      p.rec_sub[slot] = tot;
+    }
+  }
+}
+
+// H: starts the pair's share of block batch `bt` in `rows` (sorted rows, shuffle state)
+template <class RP>
+__device__ __forceinline__ void ws_start(const RP &p, const WsBatch &bt, uint32_t pair,
+                                         uint8_t *rows, FyState &f, int lane) {
+  if (static_cast<uint32_t>(lane) >= p.gen_group) return;
+  const auto &pr = p.prob[0];
+  load_dense_row(rows + static_cast<uint32_t>(lane) * p.dense_stride, pr);
+  const uint64_t cand = bt.cbeg + pair * p.gen_group + static_cast<uint32_t>(lane);
+  fy_init(f, pr, cand, cand < bt.cend && !(p.chunk_quirk && cand % p.chunk_size == 0));
+}
+
+template <int TT, int G, int D, class Source>
+__device__ __forceinline__ void ws_body(Source &src, uint8_t *lds) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = threadIdx.x >> 6;
+  WsBatch *info = reinterpret_cast<WsBatch *>(lds);
+  if (wave < kWsPairs) {  // ---- M ----
+    v4i X[G][TT];
+    v4i ring[D];
+    __syncthreads();  // batches 0 and 1 claimed
+    __syncthreads();  // batch 0 generated
+    uint32_t tick_no = 0;
+    for (uint32_t b = 0;; ++b) {
+      const WsBatch cur = info[b % 3];
+      if (!cur.valid) break;
+      const auto &p = src.params(cur.job);
+      const auto &pr = p.prob[0];
+      const uint32_t S = pr.S, CW = p.dense_cw, n = (p.gen_group / CW) * S;
+      uint8_t *base = lds + kWsHead + wave * p.ws_pair;
+      const uint8_t *rows = base + p.off_rows[b & 1];
+      uint32_t pass = 0, s = 0;
+      for (uint32_t tick = 0; tick < n; ++tick, ++tick_no) {
+        uint32_t *cnt = reinterpret_cast<uint32_t *>(base + ((tick_no & 1) ? p.off_terms
+                                                                           : p.off_hist));
+        ws_mfma_tick<TT, G, D>(pr, rows, p.dense_stride, CW, pass, s, X, ring, cnt, lane);
+        if (++s == S) s = 0, ++pass;
+        __syncthreads();
+      }
+    }
+  } else {  // ---- H ----
+    const uint32_t pair = wave - kWsPairs;
+    LevelFilter lf;
+    lf.n = 0;
+    uint32_t lf_job = 0xffffffffu;
+    FyState f{};
+    double obj = 0.0;
+    if (wave == kWsPairs) {
+      const WsBatch b0 = src.next(lane);
+      const WsBatch b1 = b0.valid ? src.next(lane) : WsBatch{0, 0, 0, 0};
+      if (lane == 0) info[0] = b0, info[1] = b1, info[2] = WsBatch{0, 0, 0, 0};
+    }
+    __syncthreads();
+    {
+      const WsBatch b0 = info[0];
+      if (b0.valid) {
+        const auto &p = src.params(b0.job);
+        uint8_t *rows = lds + kWsHead + pair * p.ws_pair + p.off_rows[0];
+        ws_start(p, b0, pair, rows, f, lane);
+        if (static_cast<uint32_t>(lane) < p.gen_group)
+          fy_steps(f, p.prob[0], rows + static_cast<uint32_t>(lane) * p.dense_stride,
+                   0xffffffffu);
+      }
+    }
+    wave_sync();
+    __syncthreads();
+    WsBatch prev{0, 0, 0, 0};
+    uint32_t tick_no = 0, prev_tick = 0;
+    // the last shell, tie filter and records of batch b - 1 (prev)
+    auto finish_prev = [&](uint32_t b) {
+      const auto &pp = src.params(prev.job);
+      uint8_t *pbase = lds + kWsHead + pair * pp.ws_pair;
+      const uint32_t pCW = pp.dense_cw;
+      ws_objective(pp.prob[0], pCW, pp.gen_group / pCW - 1, pp.prob[0].S - 1,
+                   reinterpret_cast<const uint32_t *>(
+                       pbase + ((prev_tick & 1) ? pp.off_terms : pp.off_hist)),
+                   obj, lane);
+      if (prev.job != lf_job) lf.n = 0, lf_job = prev.job;
+      ws_record(pp, pbase + pp.off_rows[(b - 1) & 1], prev.cbeg + pair * pp.gen_group,
+                prev.cend, obj, lf, lane);
+    };
+    for (uint32_t b = 0;; ++b) {
+      const WsBatch cur = info[b % 3];
+      if (!cur.valid) {
+        if (prev.valid) finish_prev(b);
+        break;
+      }
+      const WsBatch nxt = info[(b + 1) % 3];
+      const auto &p = src.params(cur.job);
+      const auto &pr = p.prob[0];
+      const uint32_t S = pr.S, CW = p.dense_cw, n = (p.gen_group / CW) * S;
+      uint8_t *base = lds + kWsHead + pair * p.ws_pair;
+      uint32_t steps = 0;
+      uint32_t pass = 0, s = 0;  // (pass, shell) of the previous tick of this batch
+      for (uint32_t tick = 0; tick < n; ++tick, ++tick_no) {
+        if (tick == 0) {
+          if (prev.valid) finish_prev(b);
+          obj = 0.0;
+          if (wave == kWsPairs) {  // claims stay two batches ahead
+            const WsBatch b2 = nxt.valid ? src.next(lane) : WsBatch{0, 0, 0, 0};
+            if (lane == 0) info[(b + 2) % 3] = b2;
+          }
+          if (nxt.valid) {
+            const auto &np = src.params(nxt.job);
+            ws_start(np, nxt, pair,
+                     lds + kWsHead + pair * np.ws_pair + np.off_rows[(b + 1) & 1], f, lane);
+            steps = (np.prob[0].draws + n - 1) / n;
+          }
+        } else {
+          ws_objective(pr, CW, pass, s,
+                       reinterpret_cast<const uint32_t *>(
+                           base + (((tick_no - 1) & 1) ? p.off_terms : p.off_hist)),
+                       obj, lane);
+          if (++s == S) s = 0, ++pass;
+        }
+        if (nxt.valid) {  // a slice of the next batch's shuffles, the rest at the last tick
+          const auto &np = src.params(nxt.job);
+          if (static_cast<uint32_t>(lane) < np.gen_group)
+            fy_steps(f, np.prob[0],
+                     lds + kWsHead + pair * np.ws_pair + np.off_rows[(b + 1) & 1] +
+                         static_cast<uint32_t>(lane) * np.dense_stride,
+                     tick + 1 == n ? 0xffffffffu : steps);
+        }
+        wave_sync();
+        __syncthreads();
+      }
+      prev = cur;
+      prev_tick = tick_no - 1;
+    }
+  }
+}
+
+template <int TT, int G, int D>
+__global__ __launch_bounds__(512, 2) void run_dense_ws_kernel(RunParams p) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  RunSource src{p};
+  ws_body<TT, G, D>(src, lds);
+}
+
+template <int TT, int G, int D>
+__global__ __launch_bounds__(512, 2) void batch_dense_ws_kernel(BatchParams bp) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  BatchSource src{bp, *bp.num_items, 0u};
+  ws_body<TT, G, D>(src, lds);
 }
 
 hipError_t dense_lds(const void *fn, size_t bytes) {
@@ -429,6 +805,15 @@ template <class Sh> hipError_t batch_dense_k(const BatchParams &bp, uint32_t blo
   hipLaunchKernelGGL((batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>), dim3(blocks), dim3(64),
                      bp.wave_lds, s, bp);
   return hipGetLastError();
+}
+
+// the paired form exists for the shapes budgeted for two wavefronts per SIMD
+template <class Sh> constexpr bool ws_shape() { return Sh::W == 2; }
+template <class Sh> constexpr int ws_ring() { return Sh::D; }
+
+template <class Sh> const void *ws_fn(bool persistent) {
+  return persistent ? reinterpret_cast<const void *>(&batch_dense_ws_kernel<Sh::TT, Sh::G, ws_ring<Sh>()>)
+                    : reinterpret_cast<const void *>(&run_dense_ws_kernel<Sh::TT, Sh::G, ws_ring<Sh>()>);
 }
 
 template <class Sh> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s) {
@@ -500,6 +885,72 @@ hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStre
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
     return score_dense_k<Sh>(p, s);
+  });
+}
+
+bool dense_paired_available(uint32_t tile_bound) {
+  bool ok = false;
+  (void)with_dense_shape(tile_bound, [&](auto sh) {
+    ok = ws_shape<decltype(sh)>();
+    return hipSuccess;
+  });
+  return ok;
+}
+
+uint32_t dense_paired_blocks_per_cu(uint32_t tile_bound, size_t block_lds, bool persistent) {
+  int n = 0;
+  const hipError_t e = with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    if constexpr (!ws_shape<Sh>()) {
+      return hipErrorInvalidValue;
+    } else {
+      const void *fn = ws_fn<Sh>(persistent);
+      hipError_t r = dense_lds(fn, block_lds);
+      if (r != hipSuccess) return r;
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * 2 * kWsPairs, block_lds);
+    }
+  });
+  return e == hipSuccess && n > 0 ? static_cast<uint32_t>(n) : 1u;
+}
+
+hipError_t launch_run_dense_paired(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
+                                   hipStream_t s) {
+  return with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    if constexpr (!ws_shape<Sh>()) {
+      return hipErrorInvalidValue;
+    } else {
+      const hipError_t e = dense_lds(ws_fn<Sh>(false), p.wave_lds);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((run_dense_ws_kernel<Sh::TT, Sh::G, ws_ring<Sh>()>), dim3(blocks),
+                         dim3(64 * 2 * kWsPairs), p.wave_lds, s, p);
+      return hipGetLastError();
+    }
+  });
+}
+
+hipError_t prepare_batch_dense_paired(uint32_t tile_bound, size_t block_lds) {
+  return with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    if constexpr (!ws_shape<Sh>()) {
+      return hipErrorInvalidValue;
+    } else {
+      return dense_lds(ws_fn<Sh>(true), block_lds);
+    }
+  });
+}
+
+hipError_t launch_batch_dense_paired(const BatchParams &bp, uint32_t tile_bound, uint32_t blocks,
+                                     hipStream_t s) {
+  return with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    if constexpr (!ws_shape<Sh>()) {
+      return hipErrorInvalidValue;
+    } else {
+      hipLaunchKernelGGL((batch_dense_ws_kernel<Sh::TT, Sh::G, ws_ring<Sh>()>), dim3(blocks),
+                         dim3(64 * 2 * kWsPairs), bp.wave_lds, s, bp);
+      return hipGetLastError();
+    }
   });
 }
 

@@ -471,6 +471,11 @@ sqsg_job *create_job(const sqsg_config &c) {
   return job.release();
 }
 
+bool paired_ok(const sqsg_job *job) {
+  return job->dense_ok && job->iteration_mode != SQSG_ITERATION_SYSTEMATIC &&
+         dense_paired_available(dense_tile_bound_of(job));
+}
+
 int resolve_kernel(const sqsg_job *job, int requested) {
   switch (requested) {
     case SQSG_KERNEL_AUTO:  // exact everywhere; fastest first where it applies
@@ -489,6 +494,11 @@ int resolve_kernel(const sqsg_job *job, int requested) {
         throw std::invalid_argument("the dense kernel needs one sublattice problem of at most "
                                     "1024 sites, 16 species and 127 upper neighbours per site "
                                     "and shell");
+      return requested;
+    case SQSG_KERNEL_DENSE_PAIRED:
+      if (!paired_ok(job))
+        throw std::invalid_argument("the paired dense kernel needs a Monte Carlo job the dense "
+                                    "kernel handles, of at most 256 sites");
       return requested;
     default:
       throw std::invalid_argument("kernel variant not available for this job");
@@ -512,6 +522,33 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   p.L = static_cast<uint32_t>(job->keep + 1);
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   uint32_t off = 0;
+  if (variant == SQSG_KERNEL_DENSE_PAIRED) {
+    // per pair: two row buffers of gen_group candidates (the batch being scored and the
+    // next one being generated) and the counts of two shells.  The
+    // layout depends on the tile bound and K only, so that the jobs of a persistent launch
+    // (grouped by both) share it.
+    const Problem &P = job->problems[0];
+    const uint32_t TT = dense_tile_bound((P.N + 31) / 32), G = dense_groups(TT);
+    const uint32_t g = std::min(32 / P.K, 64 / G);
+    p.dense_cw = G * g;
+    p.dense_stride = 32 * TT + 16;
+    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4);
+    const uint32_t budget = (160 * 1024 - kDensePairHead) / kDensePairs;
+    uint32_t npass = 64 / p.dense_cw;
+    while (npass > 1 && 2 * align(npass * p.dense_cw * p.dense_stride) + 2 * cnt_bytes > budget)
+      --npass;
+    p.gen_group = npass * p.dense_cw;
+    p.off_rows[0] = 0;
+    p.off_rows[1] = align(p.gen_group * p.dense_stride);
+    p.off_hist = 2 * p.off_rows[1];
+    p.off_terms = p.off_hist + cnt_bytes;
+    p.ws_pair = p.off_terms + cnt_bytes;
+    p.wave_lds = kDensePairHead + kDensePairs * p.ws_pair;
+    if (p.wave_lds > 160 * 1024) throw std::out_of_range("problem does not fit the LDS budget");
+    *blocks = static_cast<uint32_t>(device_cu_count()) *
+              dense_paired_blocks_per_cu(TT, p.wave_lds, false);
+    return p;
+  }
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
     // rows of gen_group candidates (npass passes of dense_cw), then one shell's counts and
     // term values
@@ -520,19 +557,15 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     const uint32_t g = std::min(32 / P.K, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
-    // one shell's counts and term values of a pass
+    // one shell's counts of a pass
     const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4);
-    const uint32_t val_bytes = align(p.dense_cw * (P.K * (P.K + 1) / 2) * 8);
     uint32_t npass = 64 / p.dense_cw;
     // at most 20 KiB per wavefront: eight resident wavefronts per CU (two per SIMD)
-    while (npass > 1 &&
-           npass * p.dense_cw * p.dense_stride + cnt_bytes + val_bytes > 20 * 1024)
-      --npass;
+    while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 20 * 1024) --npass;
     p.gen_group = npass * p.dense_cw;
     p.off_rows[0] = 0;
     p.off_hist = align(p.gen_group * p.dense_stride);
-    p.off_obj = p.off_hist + cnt_bytes;
-    p.wave_lds = p.off_obj + val_bytes;
+    p.wave_lds = p.off_hist + cnt_bytes;
     // one wavefront per block, as many as registers and LDS keep resident
     *blocks = static_cast<uint32_t>(device_cu_count()) *
               dense_waves_per_cu(dense_tile_bound(T), p.wave_lds, false);
@@ -575,6 +608,8 @@ hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, 
   if (variant == SQSG_KERNEL_SPARSE_LANE) return launch_run_lane(p, job->kmax, blocks, W, s);
   if (variant == SQSG_KERNEL_DENSE_MFMA)
     return launch_run_dense(p, dense_tile_bound_of(job), blocks, s);
+  if (variant == SQSG_KERNEL_DENSE_PAIRED)
+    return launch_run_dense_paired(p, dense_tile_bound_of(job), blocks, s);
   return launch_run_wave(p, blocks, W, s);
 }
 
@@ -642,7 +677,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   const auto t_start = clock::now();
   begin_run(job);
   const int variant = resolve_kernel(job, job->kernel);
-  if (variant == SQSG_KERNEL_DENSE_MFMA) ensure_dense(job);
+  if (variant == SQSG_KERNEL_DENSE_MFMA || variant == SQSG_KERNEL_DENSE_PAIRED) ensure_dense(job);
   const uint32_t W = variant == SQSG_KERNEL_SPARSE_WAVE ? 4 : 1;
   uint32_t blocks = 0;
   RunParams p = make_run_params(job, variant, W, &blocks);
@@ -725,8 +760,10 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
                  int32_t *d_bonds, double *d_obj, double *d_sro, int kernel, hipStream_t s) {
   if (problem >= job->problems.size()) throw std::out_of_range("problem index out of range");
   ensure_device(job);
-  const int variant = kernel == SQSG_KERNEL_AUTO ? SQSG_KERNEL_SPARSE_WAVE
-                                                 : resolve_kernel(job, kernel);
+  int variant = kernel == SQSG_KERNEL_AUTO ? SQSG_KERNEL_SPARSE_WAVE
+                                           : resolve_kernel(job, kernel);
+  // explicit scoring has no generation to overlap: the paired form scores as the dense one
+  if (variant == SQSG_KERNEL_DENSE_PAIRED) variant = SQSG_KERNEL_DENSE_MFMA;
   const Problem &P = job->problems[problem];
   ScoreParams sp{};
   sp.prob = P.dev();
@@ -754,7 +791,6 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     sp.dense_stride = rp.dense_stride;
     sp.off_row = rp.off_rows[0];
     sp.off_hist = rp.off_hist;
-    sp.off_terms = rp.off_obj;
     sp.wave_lds = rp.wave_lds;
     SQSG_HIP(launch_score_dense(sp, dense_tile_bound_of(job), s ? s : job->stream));
     return;

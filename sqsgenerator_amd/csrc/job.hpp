@@ -190,5 +190,7 @@ void ensure_device(sqsg_job *job);
 void ensure_dense(sqsg_job *job);
 // dense variant launch shape of a job: tile bound (kernel template) and 1-wave blocks
 uint32_t dense_tile_bound_of(const sqsg_job *job);
+// the paired dense form applies (Monte Carlo, a tile bound with a paired shape)
+bool paired_ok(const sqsg_job *job);
 
 }  // namespace sqsg

@@ -39,6 +39,16 @@ hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32
                               hipStream_t stream);
 hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds);
 hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStream_t stream);
+// paired form (MFMA and helper wavefronts, four pairs per block of 512 threads; MC jobs of
+// tile bounds whose shape runs two wavefronts per SIMD)
+constexpr uint32_t kDensePairs = 4, kDensePairHead = 128;  // pairs per block, LDS header
+bool dense_paired_available(uint32_t tile_bound);
+uint32_t dense_paired_blocks_per_cu(uint32_t tile_bound, size_t block_lds, bool persistent);
+hipError_t launch_run_dense_paired(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
+                                   hipStream_t stream);
+hipError_t prepare_batch_dense_paired(uint32_t tile_bound, size_t block_lds);
+hipError_t launch_batch_dense_paired(const BatchParams &bp, uint32_t tile_bound, uint32_t blocks,
+                                     hipStream_t stream);
 
 // order-preserving key of a double (unsigned order == numeric order)
 inline unsigned long long objective_key(double d) {

@@ -70,6 +70,8 @@ WORKLOADS = {
 
 KERNELS = {"wave": 1, "lane": 2, "dense": 3}
 SINGLE_PROBLEM = [n for n in WORKLOADS if not n.endswith("_split")]  # dense kernel domain
+PAIRED = ["fcc32_3shells", "fcc108_ternary", "bcc128_quaternary", "hcp128_binary",
+          "fcc256_quinary"]  # paired dense form: single problem, at most 256 sites
 
 
 @pytest.mark.parametrize("kernel", list(KERNELS))
@@ -214,7 +216,8 @@ def test_variants_agree_mc(gpu, name):
     d["iterations"] = 200_000 if name != "fcc864_quinary" else 20_000
     d["chunk_size"] = min(d["iterations"], 50_000)
     packs = []
-    for kernel in (1, 2, 3) if name in SINGLE_PROBLEM else (1, 2):
+    kernels = (1, 2, 3, 4) if name in PAIRED else (1, 2, 3) if name in SINGLE_PROBLEM else (1, 2)
+    for kernel in kernels:
         job = Job(parse_config(d))
         job.set_kernel(kernel)
         job.run()
@@ -343,7 +346,7 @@ def test_server_results_alone_and_in_the_mix(gpu):
 def test_autotune_picks_a_variant(gpu):
     job = Job(parse_config(_small("fcc256_quinary")))
     kernel, rate = job.autotune()
-    assert kernel in (1, 2, 3) and rate > 0
+    assert kernel in (1, 2, 3, 4) and rate > 0
     job.run()
     assert job.timing()[2] == 131072
 
@@ -394,3 +397,66 @@ def test_batch_dense_matches_single_job_runs(gpu, use_graph):
         assert j.timing()[2] == 131072
         assert _pack_signature(j.results()) == ref
         j.close()
+
+
+@pytest.mark.parametrize("name", PAIRED)
+@pytest.mark.parametrize("iterations", [1, 100, 241, 30_000])
+def test_paired_matches_lane_kernel(gpu, name, iterations):
+    """the paired form (MFMA and helper wavefronts, block batches of four pair batches)
+    returns the lane kernel's pack, also for ranges that leave pairs or whole blocks idle"""
+    print(f"[paired] {name} its={iterations}", flush=True)
+    packs = []
+    for kernel in (2, 4):
+        job = Job(parse_config(_small(name, iterations)))
+        job.set_kernel(kernel)
+        job.run()
+        assert job.timing()[2] == iterations
+        packs.append(_pack_signature(job.results()))
+        job.close()
+    assert packs[0] == packs[1]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_paired_matches_single_job_runs(gpu, use_graph):
+    """the persistent paired form groups jobs by tile bound and species count (one LDS
+    layout per launch); two seeds of the 256-site job share a group, jobs of other shapes
+    run in groups of their own, and every pack equals the lane kernel's"""
+    print(f"[batch-paired] graph={use_graph}", flush=True)
+    cfgs = []
+    for name in PAIRED:
+        cfgs.append(_small(name))
+    extra = _small("fcc256_quinary")
+    extra["seed"] = 4242
+    cfgs.append(extra)
+    alone = []
+    for d in cfgs:
+        job = Job(parse_config(d))
+        job.set_kernel(2)
+        job.run()
+        alone.append(_pack_signature(job.results()))
+        job.close()
+    jobs = [Job(parse_config(d)) for d in cfgs]
+    for j in jobs:
+        j.set_kernel(4)
+        j.set_launch_size(32768)
+    rep = S.batch_run(jobs, use_graph=use_graph)
+    assert rep["candidates"] == 131072 * len(jobs)
+    for j, ref in zip(jobs, alone):
+        assert j.timing()[2] == 131072
+        assert _pack_signature(j.results()) == ref
+        j.close()
+
+
+def test_paired_rejects_systematic_and_large_cells(gpu):
+    d = W.reference_test_16()
+    del d["pair_weights"]
+    job = Job(parse_config(d))
+    with pytest.raises(ValueError):
+        job.set_kernel(4)
+        job.run()
+    job.close()
+    job = Job(parse_config(_small("fcc864_quinary", 1000)))
+    with pytest.raises(ValueError):
+        job.set_kernel(4)
+        job.run()
+    job.close()
