@@ -199,8 +199,8 @@ class BatchEngine {
                             hipMemcpyHostToDevice, stream_));
     s->p.thr_key = s->buf->thr.as<unsigned long long>();
     s->footprint = footprint(*s);
-    // groups share a launch shape: the tile bound (dense), also K (paired: one LDS layout),
-    // the largest species count (lane)
+    // groups share a launch shape: the tile bound (dense), also K (paired: one LDS
+    // layout), the largest species count (lane)
     const uint32_t key = !dense ? job->kmax
                          : variant == SQSG_KERNEL_DENSE_PAIRED
                              ? dense_tile_bound_of(job) | (job->problems[0].K << 8)

@@ -104,14 +104,13 @@ __device__ __forceinline__ uint32_t tile_offset(uint32_t s, uint32_t I, uint32_t
 // are at cc (LDS), added to obj in the reference order (xi, then eta >= xi); bonds and SRO
 // cubes [S][K][K] are written when given.  The term parameters are uniform across the
 // wavefront and come through scalar loads; with K known at compile time every count read
-// has a constant offset and all loads are issued ahead of the arithmetic.
+// has a constant offset.
 template <int KT, class PR>
 __device__ __forceinline__ void shell_terms_k(const PR &pr, uint32_t s, const uint32_t *cc,
                                               double &obj, int32_t *bonds, double *sro,
                                               uint32_t Kr) {
   const uint32_t K = KT > 0 ? static_cast<uint32_t>(KT) : Kr;
-  const uint32_t Ts = K * (K + 1) / 2;
-  const TermParam *tp = pr.tparams + s * Ts;
+  const TermParam *tp = pr.tparams + s * (K * (K + 1) / 2);
   uint32_t t = 0;
 #pragma unroll
   for (uint32_t xi = 0; xi < K; ++xi)
@@ -145,7 +144,7 @@ __device__ __forceinline__ void shell_terms(const PR &pr, uint32_t s, const uint
 
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
 // bytes, padding sites 0xff); `cnt` is CW * K * K words of LDS.  Passes whose bit in `live`
-// is clear are skipped.  Returns the objective of the lane's own candidate
+// are clear are skipped.  Returns the objective of the lane's own candidate
 // (lane < npass * CW) and, when given, writes its ordered bonds and SRO cube.
 template <int TT, int G, int D, class PR>
 __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
@@ -174,6 +173,16 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
     if (col_ok && m / K == cl) keep |= 1u << r;
   }
   double obj = 0.0;
+  // a shell's terms (optimization.h:85-100), added by each candidate's own lane in the
+  // reference order
+  auto terms_of = [&](uint32_t opass, uint32_t os) {
+    if (my_pass == opass) {
+      const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
+      const bool out = (bonds || sro) && cr < out_rows;
+      shell_terms(pr, os, cnt + my_cc * KK, obj, out ? bonds + cr * cube : nullptr,
+                  out ? sro + cr * cube : nullptr);
+    }
+  };
   for (uint32_t pass = 0; pass < npass; ++pass) {
     if (!((live >> pass) & 1ull)) continue;
     v4i X[G][TT];
@@ -256,24 +265,21 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
             cnt[gi * g * KK + K * m + b] = static_cast<uint32_t>(C[gi][r]);
           }
       wave_sync();
-      // this shell's terms (optimization.h:85-100), added by each candidate's own lane in
-      // the reference order; the term parameters are uniform (scalar loads)
-      if (my_pass == pass) {
-        const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
-        const bool out = (bonds || sro) && cr < out_rows;
-        shell_terms(pr, s, cnt + my_cc * KK, obj, out ? bonds + cr * cube : nullptr,
-                    out ? sro + cr * cube : nullptr);
-      }
+      terms_of(pass, s);
       wave_sync();  // the counts are rewritten by the next shell
     }
   }
   return obj;
 }
 
-template <class PR> __device__ __forceinline__ void load_dense_row(uint8_t *row, const PR &pr) {
-  const uint32_t *src = reinterpret_cast<const uint32_t *>(pr.species0D);
-  uint32_t *dst = reinterpret_cast<uint32_t *>(row);
-  for (uint32_t w = 0; w < 8 * pr.dT; ++w) dst[w] = src[w];
+// the sorted configuration (32 * dT bytes, dT <= TT) into an LDS row, 16 bytes at a time
+template <int TT, class PR>
+__device__ __forceinline__ void load_dense_row(uint8_t *row, const PR &pr) {
+  const uint4 *src = reinterpret_cast<const uint4 *>(pr.species0D);
+  uint4 *dst = reinterpret_cast<uint4 *>(row);
+#pragma unroll
+  for (uint32_t w = 0; w < 2 * TT; ++w)
+    if (w < 2 * pr.dT) dst[w] = src[w];
 }
 
 // One segment of a job: gen_group lanes x p.rounds candidates starting at sbeg (the
@@ -297,13 +303,13 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
     if (gl) {
       if (p.systematic) {
         if (r == 0) {
-          load_dense_row(row, pr);
+          load_dense_row<TT>(row, pr);
           if (active) unrank_candidate(Strided<0>{row}, pr, cand + 1, 1u);
         } else if (active) {
           next_perm_candidate(Strided<0>{row}, pr.bounds[0], pr.bounds[1] - pr.bounds[0]);
         }
       } else {
-        load_dense_row(row, pr);
+        load_dense_row<TT>(row, pr);
         if (active && !(p.chunk_quirk && cand % p.chunk_size == 0))
           shuffle_candidate(Strided<0>{row}, pr, cand);
       }
@@ -641,12 +647,12 @@ __device__ __forceinline__ void ws_record(const RP &p, const uint8_t *rows, uint
 }
 
 // H: starts the pair's share of block batch `bt` in `rows` (sorted rows, shuffle state)
-template <class RP>
+template <int TT, class RP>
 __device__ __forceinline__ void ws_start(const RP &p, const WsBatch &bt, uint32_t pair,
                                          uint8_t *rows, FyState &f, int lane) {
   if (static_cast<uint32_t>(lane) >= p.gen_group) return;
   const auto &pr = p.prob[0];
-  load_dense_row(rows + static_cast<uint32_t>(lane) * p.dense_stride, pr);
+  load_dense_row<TT>(rows + static_cast<uint32_t>(lane) * p.dense_stride, pr);
   const uint64_t cand = bt.cbeg + pair * p.gen_group + static_cast<uint32_t>(lane);
   fy_init(f, pr, cand, cand < bt.cend && !(p.chunk_quirk && cand % p.chunk_size == 0));
 }
@@ -697,7 +703,7 @@ __device__ __forceinline__ void ws_body(Source &src, uint8_t *lds) {
       if (b0.valid) {
         const auto &p = src.params(b0.job);
         uint8_t *rows = lds + kWsHead + pair * p.ws_pair + p.off_rows[0];
-        ws_start(p, b0, pair, rows, f, lane);
+        ws_start<TT>(p, b0, pair, rows, f, lane);
         if (static_cast<uint32_t>(lane) < p.gen_group)
           fy_steps(f, p.prob[0], rows + static_cast<uint32_t>(lane) * p.dense_stride,
                    0xffffffffu);
@@ -743,7 +749,7 @@ __device__ __forceinline__ void ws_body(Source &src, uint8_t *lds) {
           }
           if (nxt.valid) {
             const auto &np = src.params(nxt.job);
-            ws_start(np, nxt, pair,
+            ws_start<TT>(np, nxt, pair,
                      lds + kWsHead + pair * np.ws_pair + np.off_rows[(b + 1) & 1], f, lane);
             steps = (np.prob[0].draws + n - 1) / n;
           }

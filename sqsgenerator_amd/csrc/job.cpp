@@ -524,9 +524,9 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   uint32_t off = 0;
   if (variant == SQSG_KERNEL_DENSE_PAIRED) {
     // per pair: two row buffers of gen_group candidates (the batch being scored and the
-    // next one being generated) and the counts of two shells.  The
-    // layout depends on the tile bound and K only, so that the jobs of a persistent launch
-    // (grouped by both) share it.
+    // next one being generated) and the counts of two shells.  The layout depends on the
+    // tile bound and K only, so that the jobs of a persistent launch (grouped by both)
+    // share it.
     const Problem &P = job->problems[0];
     const uint32_t TT = dense_tile_bound((P.N + 31) / 32), G = dense_groups(TT);
     const uint32_t g = std::min(32 / P.K, 64 / G);
@@ -550,8 +550,7 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     return p;
   }
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
-    // rows of gen_group candidates (npass passes of dense_cw), then one shell's counts and
-    // term values
+    // rows of gen_group candidates (npass passes of dense_cw), then one shell's counts
     const Problem &P = job->problems[0];
     const uint32_t T = (P.N + 31) / 32, G = dense_groups(dense_tile_bound(T));
     const uint32_t g = std::min(32 / P.K, 64 / G);
@@ -566,6 +565,7 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     p.off_rows[0] = 0;
     p.off_hist = align(p.gen_group * p.dense_stride);
     p.wave_lds = p.off_hist + cnt_bytes;
+    if (p.wave_lds > 160 * 1024) throw std::out_of_range("problem does not fit the LDS budget");
     // one wavefront per block, as many as registers and LDS keep resident
     *blocks = static_cast<uint32_t>(device_cu_count()) *
               dense_waves_per_cu(dense_tile_bound(T), p.wave_lds, false);
