@@ -238,6 +238,9 @@ def main() -> int:
             j.close()
     jobs = batch_jobs(args.steps)
     info = jobs[0].problem(0)
+    shape = jobs[0].kernel_shape()  # e.g. dense<8,2,2,9>
+    kernel_name = ("batch_dense_kernel" + shape[len("dense"):] if shape.startswith("dense")
+                   else shape)
     L.check(L.lib().sqsg_device_synchronize())
     t0 = time.perf_counter()
     rep = batch_run(jobs)
@@ -272,7 +275,7 @@ def main() -> int:
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS,
                      "traffic": measured_traffic(per_launch),
-                     "kernel": "batch_dense_kernel<8,2,2,12>", "avg_launch_ms": avg_launch_s * 1e3,
+                     "kernel": kernel_name, "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_ops_per_candidate": ops_per_cand,
                      "candidates_per_launch": per_launch,
                      "note": "int8 ops (2 per multiply-accumulate) of the dense one-hot "

@@ -8,10 +8,12 @@ from .core import (ParseError, ParseErrorCode, SqsCallbackContext, SqsConfigurat
                    SqsResultPack, Structure, distance_matrix, distances_histogram,
                    distances_naive, double, interact, num_permutations, optimize, parse_config,
                    prefactors, random, rank_permutation, shell_matrix, single, split, systematic,
-                   unrank_permutation, Job, Server, batch_run)
+                   unrank_permutation, Job, Server, batch_run, load_result_pack,
+                   StoredResultPack)
 
 __all__ = ["ParseError", "ParseErrorCode", "SqsCallbackContext", "SqsConfiguration", "SqsResult",
            "SqsResultPack", "Structure", "distance_matrix", "distances_histogram",
            "distances_naive", "double", "interact", "num_permutations", "optimize",
            "parse_config", "prefactors", "random", "rank_permutation", "shell_matrix", "single",
-           "split", "systematic", "unrank_permutation", "Job", "Server", "batch_run"]
+           "split", "systematic", "unrank_permutation", "Job", "Server", "batch_run",
+           "load_result_pack", "StoredResultPack"]

@@ -185,6 +185,16 @@ class Structure:
     def with_species(self, species) -> "Structure":
         return Structure(self.lattice, self.frac_coords, species, self.pbc)
 
+    def bytes(self) -> bytes:
+        """msgpack wire form (sqsgen.cpp:182, io/json.h:139-157)"""
+        from . import pack_io as W
+        return W.dumps(W.structure_doc(self))
+
+    @staticmethod
+    def from_bytes(data: bytes) -> "Structure":
+        from . import pack_io as W
+        return W.structure_from_doc(W.loads(bytes(data)))
+
 
 # ---- configuration ------------------------------------------------------------------------
 
@@ -217,6 +227,16 @@ class SqsConfiguration:
 
     def structure(self) -> Structure:
         return self._structure
+
+    def bytes(self) -> bytes:
+        """msgpack wire form (sqsgen.cpp:222, io/json.h:177-195)"""
+        from . import pack_io as W
+        return W.dumps(W.config_doc(self))
+
+    @staticmethod
+    def from_bytes(data: bytes) -> "SqsConfiguration":
+        from . import pack_io as W
+        return W.config_from_doc(W.loads(bytes(data)))
 
     def json(self) -> str:
         d = {
@@ -714,6 +734,12 @@ class Job:
     def results(self) -> "SqsResultPack":
         return SqsResultPack._from_job(self)
 
+    def kernel_shape(self) -> str:
+        """launch shape of the kernel a run uses, e.g. "dense<8,2,2,9>" """
+        buf = C.create_string_buffer(128)
+        L.check(L.lib().sqsg_job_kernel_shape(self.handle, buf, len(buf)))
+        return buf.value.decode()
+
     def autotune(self) -> tuple[int, float]:
         """times every kernel variant on a short run and keeps the fastest (clears results)"""
         k, r = C.c_int(), C.c_double()
@@ -945,6 +971,164 @@ class SqsResultPack:
 
     def best(self) -> SqsResult:
         return self._level(0)[0]
+
+    def _result_docs(self) -> list:
+        """the flattened results of the wire format (io/json.h:221-262), level order"""
+        from . import pack_io as W
+        if self._n == 0:
+            return []
+        arr = self.arrays(0, self._n, species=True, ranks=False)
+        sro = self.sro_array(0, self._n)
+        sub = np.empty((self._n, len(self._problems)))
+        if self.config.sublattice_mode == split:
+            for i in range(self._n):  # per-sublattice objectives
+                r = SqsResult(self, i, float(arr["objective"][i]), arr["species"][i])
+                r._fetch()
+                sub[i] = r._sub
+        shapes = [(p["num_shells"], p["num_species"]) for p in self._problems]
+        docs = []
+        for i in range(self._n):
+            cubes, off = [], 0
+            for S_, K_ in shapes:
+                cubes.append(sro[i, off:off + S_ * K_ * K_].reshape(S_, K_, K_))
+                off += S_ * K_ * K_
+            sp = arr["species"][i]
+            if self.config.sublattice_mode == split:
+                subs = [W.result_doc(sub[i, q], [int(sp[j]) for j in sl.sites], cubes[q])
+                        for q, sl in enumerate(self.config.composition)]
+                docs.append(W._obj(objective=float(arr["objective"][i]), sublattices=subs))
+            else:
+                docs.append(W.result_doc(arr["objective"][i], sp, cubes[0]))
+        return docs
+
+    def bytes(self) -> bytes:
+        """msgpack wire form of the pack (sqsgen.cpp:67-73 to_bytes, io/json.h:264-276)"""
+        from . import pack_io as W
+        return W.dumps(W.pack_doc(self.config, self.statistics, self._result_docs()))
+
+    @staticmethod
+    def from_bytes(data: bytes) -> "StoredResultPack":
+        return StoredResultPack.from_bytes(data)
+
+
+class StoredResult:
+    """A result read from the wire format: objective, original-order species (Z) and the SRO
+    cube(s) it carries; the rank is recomputed from the species (results.h:163-169)."""
+
+    def __init__(self, pack: "StoredResultPack", objective: float, species, sro, subs=None):
+        self._pack = pack
+        self.objective = objective
+        self.species = [int(z) for z in species]
+        self._sro = sro
+        self._subs = subs
+
+    def sro(self, *args):
+        if self._pack.config.sublattice_mode == split:
+            raise ValueError("split-mode results expose SRO per sublattice (.sublattices)")
+        return self._sro.copy() if not args else self.parameter(*args)
+
+    def parameter(self, shell: int, i=None, j=None):
+        shells = sorted(self._pack.config.shell_weights[0])
+        s = shells.index(shell)
+        if i is None:
+            return self._sro[s].copy()
+        zs = sorted(set(self.species))
+        return float(self._sro[s, zs.index(to_z(i)), zs.index(to_z(j))])
+
+    def rank(self) -> str:
+        from . import pack_io as W
+        if self._pack.config.sublattice_mode == split:
+            return ",".join(r.rank() for r in self.sublattices)
+        return W.species_rank(self.species)
+
+    def structure(self) -> Structure:
+        return self._pack.config.structure().with_species(self.species)
+
+    @property
+    def sublattices(self) -> list["SqsSublatticeResult"]:
+        from . import pack_io as W
+        if self._subs is None:
+            return []
+        return [SqsSublatticeResult(float(o), cube, W.species_rank(sp), {})
+                for o, sp, cube in self._subs]
+
+
+class StoredResultPack:
+    """A result pack read from the wire format (load_result_pack, sqsgen.cpp:38-56): the same
+    iteration and accessors as SqsResultPack, held in host memory."""
+
+    def __init__(self, config: SqsConfiguration, statistics: dict, results: list):
+        from . import pack_io as W
+        self.config = config
+        self.statistics = statistics
+        is_split = config.sublattice_mode == split
+        objs, levels = W.levels_from_results(results, is_split)
+        self.level_objectives = np.asarray(objs, dtype=np.float64)
+        self._levels = []
+        for docs in levels:
+            lv = []
+            for r in docs:
+                if is_split:
+                    subs = [(float(sl["objective"]), [int(z) for z in sl["species"]],
+                             W.cube_from_doc(sl["sro"])) for sl in r["sublattices"]]
+                    species = list(config.structure().species)
+                    for (_, sp, _c), sl in zip(subs, config.composition):
+                        for site, z in zip(sl.sites, sp):
+                            species[site] = z
+                    lv.append(StoredResult(self, float(r["objective"]), species, None, subs))
+                else:
+                    lv.append(StoredResult(self, float(r["objective"]), r["species"],
+                                           W.cube_from_doc(r["sro"])))
+            self._levels.append(lv)
+        self._n = sum(len(lv) for lv in self._levels)
+
+    @classmethod
+    def from_bytes(cls, data: bytes) -> "StoredResultPack":
+        from . import pack_io as W
+        doc = W.loads(bytes(data))
+        try:
+            cfg = W.config_from_doc(doc["config"])
+            stats = W.statistics_from_doc(doc["statistics"])
+            results = doc["results"]
+        except (KeyError, TypeError, ValueError) as e:
+            raise ValueError(f"Invalid result pack: {e}") from e
+        if cfg.sublattice_mode not in (interact, split):
+            raise ValueError("Invalid result pack - invalid sublattice_mode")
+        return cls(cfg, stats, results)
+
+    def bytes(self) -> bytes:
+        from . import pack_io as W
+        docs = []
+        for lv in self._levels:
+            for r in lv:
+                if r._subs is not None:
+                    docs.append(W._obj(objective=r.objective, sublattices=[
+                        W.result_doc(o, sp, cube) for o, sp, cube in r._subs]))
+                else:
+                    docs.append(W.result_doc(r.objective, r.species, r._sro))
+        return W.dumps(W.pack_doc(self.config, self.statistics, docs))
+
+    def __iter__(self):
+        for l, lv in enumerate(self._levels):
+            yield float(self.level_objectives[l]), list(lv)
+
+    def __len__(self) -> int:
+        return len(self._levels)
+
+    def num_results(self) -> int:
+        return self._n
+
+    def best(self) -> StoredResult:
+        return self._levels[0][0]
+
+
+def load_result_pack(data, prec: str = double) -> StoredResultPack:
+    """python/extension/sqsgen.cpp:38-56,528-531: a msgpack result pack from a file path or
+    from the bytes themselves"""
+    from . import pack_io as W
+    if prec != double:
+        raise ValueError("this backend keeps results in FP64 only (prec \"double\")")
+    return StoredResultPack.from_bytes(W.read_pack_input(data))
 
 
 def optimize(config: Union[dict, SqsConfiguration, str], level: Any = None,

@@ -370,6 +370,18 @@ int sqsg_batch_run(sqsg_job **jobs, uint32_t num_jobs, int use_graph, sqsg_batch
   });
 }
 
+int sqsg_job_kernel_shape(sqsg_job *job, char *buf, size_t len) {
+  return guard([&] {
+    need(job, "job");
+    need(buf, "buf");
+    if (len == 0) throw std::invalid_argument("empty buffer");
+    const std::string s = sqsg::kernel_shape(job);
+    const size_t n = std::min(len - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = '\0';
+  });
+}
+
 int sqsg_autotune(sqsg_job *job, int *chosen, double *cand_per_s) {
   return guard([&] {
     need(job, "job");

@@ -894,6 +894,18 @@ hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStre
   });
 }
 
+std::string dense_shape_string(uint32_t tile_bound, bool paired) {
+  std::string s;
+  (void)with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    s = std::to_string(Sh::TT) + "," + std::to_string(Sh::G) + ",";
+    s += paired ? std::to_string(ws_ring<Sh>())
+                : std::to_string(Sh::W) + "," + std::to_string(Sh::D);
+    return hipSuccess;
+  });
+  return s;
+}
+
 bool dense_paired_available(uint32_t tile_bound) {
   bool ok = false;
   (void)with_dense_shape(tile_bound, [&](auto sh) {

@@ -272,9 +272,9 @@ uint32_t dense_tile_bound_of(const sqsg_job *job) {
   return dense_tile_bound((job->problems[0].N + 31) / 32);
 }
 
-void ensure_device(sqsg_job *job) {
-  if (job->on_device) return;
-  for (auto &P : job->problems) P.upload_device();
+// kernel applicability (host data only)
+void analyze_job(sqsg_job *job) {
+  if (job->analyzed) return;
   job->kmax = 0;
   for (const auto &P : job->problems) job->kmax = std::max(job->kmax, P.K);
   job->lane_ok = job->kmax <= 8;
@@ -285,8 +285,6 @@ void ensure_device(sqsg_job *job) {
     for (size_t c : per_shell)
       if (c >= 65536) job->lane_ok = false;  // 16-bit histogram fields
   }
-  if (job->lane_ok)
-    for (auto &P : job->problems) P.build_lane(lane_field_bits(job->kmax));
   for (auto &P : job->problems) {
     std::vector<uint32_t> deg(size_t(P.S) * P.N, 0);
     P.max_upper = 0;
@@ -295,6 +293,15 @@ void ensure_device(sqsg_job *job) {
   }
   job->dense_ok = job->problems.size() == 1 && job->problems[0].N <= 1024 &&
                   job->problems[0].K <= 16 && job->problems[0].max_upper <= 127;
+  job->analyzed = true;
+}
+
+void ensure_device(sqsg_job *job) {
+  if (job->on_device) return;
+  analyze_job(job);
+  for (auto &P : job->problems) P.upload_device();
+  if (job->lane_ok)
+    for (auto &P : job->problems) P.build_lane(lane_field_bits(job->kmax));
   SQSG_HIP(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking));
   SQSG_HIP(hipEventCreate(&job->ev0));
   SQSG_HIP(hipEventCreate(&job->ev1));
@@ -469,6 +476,20 @@ sqsg_job *create_job(const sqsg_config &c) {
       throw std::out_of_range("iterations exceed the number of permutations");
   }
   return job.release();
+}
+
+std::string kernel_shape(sqsg_job *job) {
+  analyze_job(job);
+  switch (resolve_kernel(job, job->kernel)) {
+    case SQSG_KERNEL_DENSE_MFMA:
+      return "dense<" + dense_shape_string(dense_tile_bound_of(job), false) + ">";
+    case SQSG_KERNEL_DENSE_PAIRED:
+      return "paired<" + dense_shape_string(dense_tile_bound_of(job), true) + ">";
+    case SQSG_KERNEL_SPARSE_LANE:
+      return "lane<" + std::to_string(job->kmax) + ">";
+    default:
+      return "wave";
+  }
 }
 
 bool paired_ok(const sqsg_job *job) {

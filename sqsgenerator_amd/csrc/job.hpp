@@ -124,6 +124,7 @@ struct sqsg_job {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool on_device = false;
+  bool analyzed = false;  // kmax, lane_ok, dense_ok set
   uint32_t kmax = 0;      // largest species count over the problems
   bool lane_ok = false;   // the lane-per-candidate kernel applies
   bool dense_ok = false;  // the dense MFMA contraction applies
@@ -192,5 +193,7 @@ void ensure_dense(sqsg_job *job);
 uint32_t dense_tile_bound_of(const sqsg_job *job);
 // the paired dense form applies (Monte Carlo, a tile bound with a paired shape)
 bool paired_ok(const sqsg_job *job);
+// launch shape of the kernel a run of the job uses ("dense<TT,G,W,D>", "lane<K>", ...)
+std::string kernel_shape(sqsg_job *job);
 
 }  // namespace sqsg

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <string>
+
 #include "device_params.hpp"
 
 namespace sqsg {
@@ -43,6 +45,8 @@ hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStre
 // tile bounds whose shape runs two wavefronts per SIMD)
 constexpr uint32_t kDensePairs = 4, kDensePairHead = 128;  // pairs per block, LDS header
 bool dense_paired_available(uint32_t tile_bound);
+// "TT,G,W,D" of the dense shape of a tile bound (paired: "TT,G,D")
+std::string dense_shape_string(uint32_t tile_bound, bool paired);
 uint32_t dense_paired_blocks_per_cu(uint32_t tile_bound, size_t block_lds, bool persistent);
 hipError_t launch_run_dense_paired(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
                                    hipStream_t stream);

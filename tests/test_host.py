@@ -182,3 +182,32 @@ def test_seed_single_thread_reproducible_config():
     d = W.reference_test_16(iteration_mode="random", iterations=500, seed=42)
     a, b = parse_config(d), parse_config(d)
     assert a.seed == b.seed == [42]
+
+
+def test_kernel_shape_follows_the_applicability_rules():
+    """the launch shape a run would use, resolved on the host (no device needed): AUTO takes
+    the dense form for one problem of at most 1024 sites; the paired form takes Monte Carlo
+    jobs of at most 256 sites; the lane kernel is sized by K"""
+    from sqsgenerator_amd.core import Job
+    job = Job(parse_config(W.quinary_256(iterations=1000)))
+    assert job.kernel_shape() == "dense<8,2,2,9>"
+    job.set_kernel(4)
+    assert job.kernel_shape() == "paired<8,2,9>"
+    job.set_kernel(2)
+    assert job.kernel_shape() == "lane<5>"
+    job.set_kernel(1)
+    assert job.kernel_shape() == "wave"
+    job.close()
+    big = Job(parse_config(W.quinary_864(iterations=1000)))
+    assert big.kernel_shape().startswith("dense<28,")
+    big.set_kernel(4)
+    with pytest.raises(ValueError):
+        big.kernel_shape()
+    big.close()
+    sys16 = W.reference_test_16()
+    del sys16["pair_weights"]
+    syst = Job(parse_config(sys16))
+    syst.set_kernel(4)
+    with pytest.raises(ValueError):
+        syst.kernel_shape()
+    syst.close()
