@@ -75,13 +75,25 @@ def cpu_baseline(job, budget_s: float) -> dict:
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     st, _ = O.run([prob], 2000, chunk_size=2000, threads=1, collect=False)
     rate1 = st.finished / max(st.seconds, 1e-9)
+    # one thread, about a tenth of the budget
+    n1 = max(2000, int(rate1 * budget_s / 10))
+    s1, _ = O.run([prob], n1, chunk_size=max(1000, n1 // 8), threads=1, collect=False)
     n = max(threads * 1000, int(rate1 * threads * budget_s))
     chunk = max(1000, n // (threads * 8))
     st, _ = O.run([prob], n, chunk_size=chunk, threads=threads, shared_rng=True, collect=False)
+    # per-thread RNG streams (no shared state): the stronger CPU figure, a third of the budget
+    n3 = max(threads * 1000, n // 3)
+    s3, _ = O.run([prob], n3, chunk_size=max(1000, n3 // (threads * 8)), threads=threads,
+                  shared_rng=False, collect=False)
     return {"value": st.finished / st.seconds, "unit": "candidates/s", "cores": threads,
             "kind": "port",
             "sample": f"{st.finished} MC candidates of the same 256-site job, {threads} threads, "
-                      f"shared RNG state, {st.seconds:.1f} s on {cpu_model()}"}
+                      f"shared RNG state, {st.seconds:.1f} s on {cpu_model()}",
+            "one_thread": {"value": s1.finished / s1.seconds, "cores": 1,
+                           "sample": f"{s1.finished} candidates, {s1.seconds:.1f} s"},
+            "per_thread_rng": {"value": s3.finished / s3.seconds, "cores": threads,
+                               "sample": f"{s3.finished} candidates, {threads} threads with "
+                                         f"their own RNG streams, {s3.seconds:.1f} s"}}
 
 
 def serving_jobs(n, seed, max_budget):
@@ -175,6 +187,10 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-max-budget", type=float, default=1e7)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-sweep", default="10,30,100,300,1000",
+                    help="arrival rates (jobs/s) of the serving sweep; empty: none")
+    ap.add_argument("--serve-sweep-jobs", type=int, default=240,
+                    help="requests per sweep rate at most (the window shrinks with the rate)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -285,6 +301,14 @@ def main() -> int:
     if not args.no_serve:
         out["serving"] = serving(args.serve_rate, args.serve_seconds, 20261203,
                                  args.serve_max_budget)
+        sweep = []
+        for r in (float(x) for x in args.serve_sweep.split(",") if x.strip()):
+            window = min(args.serve_seconds, args.serve_sweep_jobs / r)
+            res = serving(r, window, 20261203, args.serve_max_budget)
+            res.pop("mix")
+            sweep.append(res)
+        if sweep:
+            out["serving_sweep"] = sweep
     if not args.no_cpu:
         cj = make(per_step)
         out["cpu_baseline"] = cpu_baseline(cj, args.cpu_budget)
