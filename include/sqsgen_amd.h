@@ -180,6 +180,13 @@ typedef struct sqsg_statistics {   /* types.h:197-206 */
 typedef int (*sqsg_progress_fn)(const sqsg_statistics *stats, void *user);
 SQSG_API int sqsg_job_set_kernel(sqsg_job *job, int kernel);
 SQSG_API int sqsg_job_set_launch_size(sqsg_job *job, uint64_t candidates_per_launch);
+/* Early exit (no reference counterpart; for callers that want time to solution): with every
+ * scaled pair weight >= 0 each objective term is >= 0, so a candidate whose partial objective
+ * after some shell already exceeds the job's keep threshold can never be kept.  When enabled,
+ * the dense kernels stop a pass of candidates at the first shell after which none of them can
+ * be kept.  Result packs are identical; scored counts still include every candidate.  Rejected
+ * (SQSG_ERR_INVALID_ARGUMENT) when a pair weight is negative; other kernels score in full. */
+SQSG_API int sqsg_job_set_pruning(sqsg_job *job, int enable);
 SQSG_API int sqsg_run(sqsg_job *job, sqsg_progress_fn cb, void *user);
 /* cooperative cancellation, async-safe (sqs.h:385-396) */
 SQSG_API void sqsg_cancel(sqsg_job *job);

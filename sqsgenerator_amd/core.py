@@ -698,6 +698,11 @@ class Job:
     def set_kernel(self, kernel: int) -> None:
         L.check(L.lib().sqsg_job_set_kernel(self.handle, kernel))
 
+    def set_pruning(self, enable: bool = True) -> None:
+        """early exit of passes that cannot reach the kept levels (identical results; needs
+        non-negative pair weights)"""
+        L.check(L.lib().sqsg_job_set_pruning(self.handle, int(enable)))
+
     def set_launch_size(self, n: int) -> None:
         L.check(L.lib().sqsg_job_set_launch_size(self.handle, n))
 

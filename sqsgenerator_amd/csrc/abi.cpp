@@ -268,6 +268,20 @@ int sqsg_job_set_kernel(sqsg_job *job, int kernel) {
   });
 }
 
+int sqsg_job_set_pruning(sqsg_job *job, int enable) {
+  return guard([&] {
+    need(job, "job");
+    if (enable)
+      for (const auto &P : job->problems)
+        for (uint32_t t : P.terms) {
+          const uint32_t s = t & 0xff, xi = (t >> 8) & 0xff, eta = t >> 16;
+          if (!(P.pw[(s * P.K + xi) * P.K + eta] >= 0.0))
+            throw std::invalid_argument("pruning needs non-negative pair weights");
+        }
+    job->prune = enable != 0;
+  });
+}
+
 int sqsg_job_set_launch_size(sqsg_job *job, uint64_t n) {
   return guard([&] {
     need(job, "job");

@@ -144,14 +144,19 @@ __device__ __forceinline__ void shell_terms(const PR &pr, uint32_t s, const uint
 
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
 // bytes, padding sites 0xff); `cnt` is CW * K * K words of LDS.  Passes whose bit in `live`
-// are clear are skipped.  Returns the objective of the lane's own candidate
+// are clear are skipped.  With `prune_key` below ~0, a pass stops after the first shell at
+// which no active candidate of it (bit set in `active`) has a partial objective key at or
+// below prune_key: every term is >= 0 (the host checks the pair weights), so partial sums
+// only grow and such candidates can never be kept.  Returns the objective of the lane's own candidate
 // (lane < npass * CW) and, when given, writes its ordered bonds and SRO cube.
 template <int TT, int G, int D, class PR>
 __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
                                               uint32_t stride, uint32_t *cnt,
                                               uint32_t CW, uint32_t npass,
                                               unsigned long long live, int lane, int32_t *bonds,
-                                              double *sro, uint32_t out_rows) {
+                                              double *sro, uint32_t out_rows,
+                                              unsigned long long active = 0,
+                                              unsigned long long prune_key = ~0ull) {
   constexpr int P = TT * (TT + 1) / 2;
   const uint32_t K = pr.K, KK = K * K, T = pr.dT, S = pr.S;
   const uint32_t g = CW / G;
@@ -267,6 +272,11 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
       wave_sync();
       terms_of(pass, s);
       wave_sync();  // the counts are rewritten by the next shell
+      if (prune_key != ~0ull && s + 1 < S) {
+        const bool alive = my_pass == pass && ((active >> lane) & 1ull) &&
+                           okey(obj) <= prune_key;
+        if (__ballot(alive) == 0) break;
+      }
     }
   }
   return obj;
@@ -320,9 +330,18 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
     for (uint32_t q = 0; q < npass; ++q)
       if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
     wave_sync();
+    // early exit bound: the job threshold and the wavefront's own L-th level as they stand
+    // now (both only decrease, so they stay valid bounds for the record test below)
+    unsigned long long prune_key = ~0ull;
+    if (p.prune) {
+      const unsigned long long g0 =
+          __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long l0 = lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
+      prune_key = g0 < l0 ? g0 : l0;
+    }
     const double tot =
         dense_score<TT, G, D>(pr, rows, stride, cnt, CW, npass, live, lane, nullptr, nullptr,
-                              0u);
+                              0u, am, prune_key);
     scored += static_cast<unsigned long long>(__popcll(am));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -71,6 +71,7 @@ struct RunParams {
   uint32_t off_rows[kMaxProblems];
   uint32_t off_hist, off_terms, off_obj;  // dense: off_hist holds one shell's counts
   uint32_t ws_pair;            // dense paired form: LDS bytes of one pair's region
+  uint32_t prune;              // dense: stop a pass once no candidate can reach the kept levels
   // tie-keeping threshold and record sink
   unsigned long long *thr_key;
   unsigned int *rec_count;

@@ -541,6 +541,7 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   p.chunk_quirk = (!p.systematic && job->sublattice_mode == SQSG_SUBLATTICE_INTERACT) ? 1u : 0u;
   p.chunk_size = job->chunk_size;
   p.L = static_cast<uint32_t>(job->keep + 1);
+  p.prune = job->prune ? 1u : 0u;
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   uint32_t off = 0;
   if (variant == SQSG_KERNEL_DENSE_PAIRED) {

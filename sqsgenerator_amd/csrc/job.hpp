@@ -120,6 +120,7 @@ struct sqsg_job {
   uint64_t iterations = 0, chunk_size = 0, keep = 1;
   int64_t max_results = -1;
   int kernel = SQSG_KERNEL_AUTO;
+  bool prune = false;     // early exit of hopeless passes (dense kernel, pw >= 0)
   uint64_t launch_size = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

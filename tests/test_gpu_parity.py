@@ -460,3 +460,57 @@ def test_paired_rejects_systematic_and_large_cells(gpu):
         job.set_kernel(4)
         job.run()
     job.close()
+
+
+@pytest.mark.parametrize("name", ["fcc256_quinary", "fcc108_ternary", "bcc128_quaternary",
+                                  "hcp128_binary", "fcc32_3shells", "fcc864_quinary"])
+def test_pruned_dense_matches_lane_kernel(gpu, name):
+    """early exit of hopeless passes (dense kernel, non-negative pair weights) keeps the
+    result pack bit-identical and still counts every candidate"""
+    print(f"[pruned] {name}", flush=True)
+    its = 200_000 if name != "fcc864_quinary" else 20_000
+    ref = Job(parse_config(_small(name, its)))
+    ref.set_kernel(2)
+    ref.run()
+    want = _pack_signature(ref.results())
+    ref.close()
+    job = Job(parse_config(_small(name, its)))
+    job.set_kernel(3)
+    job.set_pruning(True)
+    job.set_launch_size(32768)
+    job.run()
+    assert job.timing()[2] == its
+    assert _pack_signature(job.results()) == want
+    job.close()
+
+
+def test_pruned_batch_and_systematic(gpu):
+    names = ["fcc256_quinary", "fcc108_ternary", "hcp128_binary"]
+    alone = []
+    for name in names:
+        j = Job(parse_config(_small(name)))
+        j.set_kernel(2)
+        j.run()
+        alone.append(_pack_signature(j.results()))
+        j.close()
+    jobs = [Job(parse_config(_small(name))) for name in names]
+    for j in jobs:
+        j.set_kernel(3)
+        j.set_pruning(True)
+        j.set_launch_size(32768)
+    S.batch_run(jobs, use_graph=True)
+    for j, want in zip(jobs, alone):
+        assert _pack_signature(j.results()) == want
+        j.close()
+    d = W.reference_test_16()
+    del d["pair_weights"]
+    packs = []
+    for prune in (False, True):
+        j = Job(parse_config(d))
+        j.set_kernel(3)
+        j.set_pruning(prune)
+        j.set_launch_size(1000)
+        j.run()
+        packs.append(_pack_signature(j.results()))
+        j.close()
+    assert packs[0] == packs[1]

@@ -211,3 +211,20 @@ def test_kernel_shape_follows_the_applicability_rules():
     with pytest.raises(ValueError):
         syst.kernel_shape()
     syst.close()
+
+
+def test_pruning_needs_non_negative_pair_weights():
+    from sqsgenerator_amd.core import Job
+    job = Job(parse_config(W.quinary_256(iterations=1000)))
+    job.set_pruning(True)
+    job.set_pruning(False)
+    job.close()
+    d = W.reference_test_16()
+    d["pair_weights"] = [[0.0, -1.0], [-1.0, 0.0]]
+    cfg = parse_config(d)
+    assert not isinstance(cfg, ParseError), cfg
+    job = Job(cfg)
+    with pytest.raises(ValueError):
+        job.set_pruning(True)
+    job.set_pruning(False)
+    job.close()
