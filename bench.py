@@ -96,6 +96,62 @@ def cpu_baseline(job, budget_s: float) -> dict:
                                          f"their own RNG streams, {s3.seconds:.1f} s"}}
 
 
+def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
+    """explicit scoring of `batch` uniformly random candidates per launch (seed 7) on the FCC
+    4x4x4 (256 sites) and 6x6x6 (864 sites) quinary 7-shell cells, by the sparse lane kernel
+    and the dense MFMA kernel; device-resident buffers, events on the launch stream; bond
+    counts of the two kernels compared bit for bit"""
+    import ctypes as C
+    import numpy as np
+    import torch
+    import sqsgenerator_amd._lib as L
+    import workloads as W
+    from sqsgenerator_amd.core import Job, parse_config
+
+    out = {}
+    for name, make in (("fcc256", W.quinary_256), ("fcc864", W.quinary_864)):
+        job = Job(parse_config(make(iterations=batch)))
+        info = job.problem(0)
+        N, K, S = info["num_sites"], info["num_species"], info["num_shells"]
+        rng = np.random.default_rng(7)
+        cands = np.repeat(info["species_packed"][None, :], batch, axis=0)
+        for b in range(batch):
+            for lo, hi in info["bounds"]:
+                cands[b, lo:hi] = rng.permutation(cands[b, lo:hi])
+        dev = torch.device("cuda:0")
+        sp = torch.from_numpy(cands).to(dev)
+        res = {}
+        bonds = {}
+        # a stream of our own (default priority): the library maps a NULL stream to the job's
+        # stream, and torch's default stream handle is NULL
+        stream = torch.cuda.Stream(device=dev)
+        for kname, kernel in (("lane", L.KERNEL_SPARSE_LANE), ("dense", L.KERNEL_DENSE_MFMA)):
+            bo = torch.empty((batch, S, K, K), dtype=torch.int32, device=dev)
+            ob = torch.empty(batch, dtype=torch.float64, device=dev)
+
+            def launch():
+                L.check(L.lib().sqsg_score_batch_device(
+                    job.handle, 0, C.c_void_p(sp.data_ptr()), batch, C.c_void_p(bo.data_ptr()),
+                    C.c_void_p(ob.data_ptr()), None, kernel, C.c_void_p(stream.cuda_stream)))
+            torch.cuda.synchronize(dev)  # inputs and outputs allocated on the default stream
+            launch()
+            torch.cuda.synchronize(dev)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(launches):
+                launch()
+            e1.record(stream)
+            torch.cuda.synchronize(dev)
+            ms = e0.elapsed_time(e1) / launches
+            res[kname] = {"candidates_per_s": batch / (ms * 1e-3), "ms_per_launch": ms}
+            bonds[kname] = bo.cpu()
+        res["bonds_identical"] = bool(torch.equal(bonds["lane"], bonds["dense"]))
+        res["sites"], res["dense_shape"] = N, job.kernel_shape()
+        job.close()
+        out[name] = res
+    return out
+
+
 def serving_jobs(n, seed, max_budget):
     """request mix of the serving stream: FCC/BCC/HCP cells of 32-864 sites, 2-6 species
     with random integer compositions, 2-7 shells (weights 1/s), log-uniform budgets"""
@@ -187,6 +243,10 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-max-budget", type=float, default=1e7)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--large-cells", action="store_true",
+                    help="only the large-cell sparse-vs-dense scoring (4096 candidates per launch)")
+    ap.add_argument("--large-cell-launches", type=int, default=10)
+    ap.add_argument("--no-large-cells", action="store_true")
     ap.add_argument("--no-prune", action="store_true",
                     help="skip the secondary run with early exit (sqsg_job_set_pruning)")
     ap.add_argument("--serve-sweep", default="10,30,100,300,1000",
@@ -218,6 +278,10 @@ def main() -> int:
     import workloads as W
     from sqsgenerator_amd.core import Job, batch_run, parse_config
 
+    if args.large_cells:
+        print(json.dumps({"large_cell_scoring": large_cell_scoring(args.large_cell_launches)}),
+              flush=True)
+        return 0
     per_step = args.candidates_per_step
     nb = args.batch_jobs
 
@@ -319,6 +383,8 @@ def main() -> int:
                              "contraction S(2N^2K + 2NK^2) per candidate; launch time from HIP "
                              "events around the graph launch on the engine's stream"},
     }
+    if not args.no_large_cells:
+        out["large_cell_scoring"] = large_cell_scoring(args.large_cell_launches)
     if not args.no_serve:
         out["serving"] = serving(args.serve_rate, args.serve_seconds, 20261203,
                                  args.serve_max_budget)
