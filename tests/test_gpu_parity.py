@@ -514,3 +514,56 @@ def test_pruned_batch_and_systematic(gpu):
         packs.append(_pack_signature(j.results()))
         j.close()
     assert packs[0] == packs[1]
+
+
+def _cell(name):
+    """cells that exercise the remaining dense shapes: tile bound 16 (500 sites), 32 (1024
+    sites, the largest) and 16 species (the dense kernel's maximum, beyond the lane kernel)"""
+    metals16 = ["Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Zr", "Nb", "Mo", "Ru",
+                "Rh", "Pd", "Ag"]
+    if name == "fcc500_quinary":
+        comp = {"Fe": 100, "Co": 100, "Ni": 100, "Cr": 100, "Mn": 100}
+        return W.fcc(4.05, (5, 5, 5), comp, 4, seed=71, thread_config=1)
+    if name == "fcc1024_ternary":
+        comp = {"Fe": 342, "Co": 341, "Ni": 341}
+        return W.fcc(4.05, (8, 8, 4), comp, 3, seed=72, thread_config=1)
+    if name == "fcc256_16species":
+        comp = {m: 16 for m in metals16}
+        return W.fcc(4.05, (4, 4, 4), comp, 3, seed=73, thread_config=1)
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name,ref_kernel", [("fcc500_quinary", 2), ("fcc1024_ternary", 2),
+                                             ("fcc256_16species", 1)])
+def test_dense_shapes_match_sparse(gpu, name, ref_kernel):
+    """explicit scoring of random candidates (bit-exact bonds, objectives and SRO against the
+    sparse kernel, bonds against the oracle on a sample) and a short MC run whose pack equals
+    the sparse kernel's"""
+    print(f"[dense-shapes] {name}", flush=True)
+    cfg = parse_config(_cell(name))
+    assert not isinstance(cfg, S.ParseError), cfg
+    job = Job(cfg)
+    ref = oracle_inputs(cfg, job)
+    rng = np.random.default_rng(11)
+    cands = random_candidates(ref["packed"], ref["bounds"], 512, rng)
+    b_ref, o_ref, s_ref = job.score(cands, 0, kernel=ref_kernel)
+    b_d, o_d, s_d = job.score(cands, 0, kernel=3)
+    assert np.array_equal(b_ref, b_d)
+    assert np.array_equal(o_ref.view(np.uint64), o_d.view(np.uint64))
+    assert np.array_equal(s_ref, s_d)
+    for b in rng.choice(512, 8, replace=False):
+        rb = O.count_bonds(ref["pairs"], cands[b], ref["S"], ref["K"])
+        assert np.array_equal(b_d[b].astype(np.uint64), rb)
+    job.close()
+    packs = []
+    for kernel in (ref_kernel, 3):
+        d = _cell(name)
+        d["iterations"] = 20_000
+        d["chunk_size"] = 20_000
+        j = Job(parse_config(d))
+        j.set_kernel(kernel)
+        j.run()
+        assert j.timing()[2] == 20_000
+        packs.append(_pack_signature(j.results()))
+        j.close()
+    assert packs[0] == packs[1]
