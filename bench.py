@@ -96,6 +96,47 @@ def cpu_baseline(job, budget_s: float) -> dict:
                                          f"their own RNG streams, {s3.seconds:.1f} s"}}
 
 
+class _Hip:
+    """the few HIP runtime calls the explicit-scoring measurement needs (device buffers, a
+    stream of its own, events), through ctypes"""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        try:
+            self.h = C.CDLL("libamdhip64.so")
+        except OSError:
+            self.h = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def check(self, e, what):
+        if e != 0:
+            raise RuntimeError(f"{what} failed with HIP error {e}")
+
+    def malloc(self, n):
+        p = self.C.c_void_p()
+        self.check(self.h.hipMalloc(self.C.byref(p), self.C.c_size_t(n)), "hipMalloc")
+        return p
+
+    def upload(self, dst, arr):
+        self.check(self.h.hipMemcpy(dst, arr.ctypes.data_as(self.C.c_void_p),
+                                    self.C.c_size_t(arr.nbytes), 1), "hipMemcpy")
+
+    def download(self, arr, src):
+        self.check(self.h.hipMemcpy(arr.ctypes.data_as(self.C.c_void_p), src,
+                                    self.C.c_size_t(arr.nbytes), 2), "hipMemcpy")
+
+    def stream(self):
+        s = self.C.c_void_p()
+        self.check(self.h.hipStreamCreateWithFlags(self.C.byref(s), 1), "hipStreamCreate")
+        return s
+
+    def event(self):
+        e = self.C.c_void_p()
+        self.check(self.h.hipEventCreate(self.C.byref(e)), "hipEventCreate")
+        return e
+
+
 def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
     """explicit scoring of `batch` uniformly random candidates per launch (seed 7) on the FCC
     4x4x4 (256 sites) and 6x6x6 (864 sites) quinary 7-shell cells, by the sparse lane kernel
@@ -103,11 +144,13 @@ def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
     counts of the two kernels compared bit for bit"""
     import ctypes as C
     import numpy as np
-    import torch
     import sqsgenerator_amd._lib as L
     import workloads as W
     from sqsgenerator_amd.core import Job, parse_config
 
+    hip = _Hip()
+    stream = hip.stream()  # default priority; a NULL stream would mean the job's own stream
+    e0, e1 = hip.event(), hip.event()
     out = {}
     for name, make in (("fcc256", W.quinary_256), ("fcc864", W.quinary_864)):
         job = Job(parse_config(make(iterations=batch)))
@@ -118,37 +161,39 @@ def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
         for b in range(batch):
             for lo, hi in info["bounds"]:
                 cands[b, lo:hi] = rng.permutation(cands[b, lo:hi])
-        dev = torch.device("cuda:0")
-        sp = torch.from_numpy(cands).to(dev)
-        res = {}
-        bonds = {}
-        # a stream of our own (default priority): the library maps a NULL stream to the job's
-        # stream, and torch's default stream handle is NULL
-        stream = torch.cuda.Stream(device=dev)
+        cands = np.ascontiguousarray(cands, dtype=np.uint8)
+        d_sp = hip.malloc(cands.nbytes)
+        hip.upload(d_sp, cands)
+        d_bo = hip.malloc(batch * S * K * K * 4)
+        d_ob = hip.malloc(batch * 8)
+        res, bonds = {}, {}
         for kname, kernel in (("lane", L.KERNEL_SPARSE_LANE), ("dense", L.KERNEL_DENSE_MFMA)):
-            bo = torch.empty((batch, S, K, K), dtype=torch.int32, device=dev)
-            ob = torch.empty(batch, dtype=torch.float64, device=dev)
-
             def launch():
-                L.check(L.lib().sqsg_score_batch_device(
-                    job.handle, 0, C.c_void_p(sp.data_ptr()), batch, C.c_void_p(bo.data_ptr()),
-                    C.c_void_p(ob.data_ptr()), None, kernel, C.c_void_p(stream.cuda_stream)))
-            torch.cuda.synchronize(dev)  # inputs and outputs allocated on the default stream
+                L.check(L.lib().sqsg_score_batch_device(job.handle, 0, d_sp, batch, d_bo, d_ob,
+                                                        None, kernel, stream))
             launch()
-            torch.cuda.synchronize(dev)
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
+            hip.check(hip.h.hipStreamSynchronize(stream), "hipStreamSynchronize")
+            hip.check(hip.h.hipEventRecord(e0, stream), "hipEventRecord")
             for _ in range(launches):
                 launch()
-            e1.record(stream)
-            torch.cuda.synchronize(dev)
-            ms = e0.elapsed_time(e1) / launches
-            res[kname] = {"candidates_per_s": batch / (ms * 1e-3), "ms_per_launch": ms}
-            bonds[kname] = bo.cpu()
-        res["bonds_identical"] = bool(torch.equal(bonds["lane"], bonds["dense"]))
+            hip.check(hip.h.hipEventRecord(e1, stream), "hipEventRecord")
+            hip.check(hip.h.hipEventSynchronize(e1), "hipEventSynchronize")
+            ms = C.c_float()
+            hip.check(hip.h.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+            per = ms.value / launches
+            res[kname] = {"candidates_per_s": batch / (per * 1e-3), "ms_per_launch": per}
+            host = np.empty((batch, S, K, K), dtype=np.int32)
+            hip.download(host, d_bo)
+            bonds[kname] = host
+        res["bonds_identical"] = bool(np.array_equal(bonds["lane"], bonds["dense"]))
         res["sites"], res["dense_shape"] = N, job.kernel_shape()
+        for p in (d_sp, d_bo, d_ob):
+            hip.h.hipFree(p)
         job.close()
         out[name] = res
+    for e in (e0, e1):
+        hip.h.hipEventDestroy(e)
+    hip.h.hipStreamDestroy(stream)
     return out
 
 

@@ -70,10 +70,12 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T &constant_r
 // launch shapes: tile bound TT, 32-column tile groups G per wavefront (X holds G * TT
 // one-hot tiles in registers), resident wavefronts W per SIMD the registers are budgeted
 // for, and the tile-stream prefetch depth D (grid positions; it must divide the
-// TT (TT + 1) / 2 positions of a shell so every shell reuses the same ring slots).  With
-// two wavefronts per SIMD (256 registers each), one generates candidates or sums
-// objectives while the other keeps the matrix core busy; large tile bounds need the whole
-// register file for the one-hot tiles and the ring.
+// TT (TT + 1) / 2 positions of a shell so every shell reuses the same ring slots).  Every
+// shape runs two wavefronts per SIMD (256 registers each): while one generates candidates,
+// sums objectives or waits on the tile stream, the other keeps the matrix core busy.  For
+// the large tile bounds the one-hot tiles take most of the registers, so their rings are
+// short (measured: a second wavefront per SIMD beats a deeper ring by 33-58% at 500-1024
+// sites).
 template <int TT_, int G_, int W_, int D_> struct DenseShape {
   static constexpr int TT = TT_, G = G_, W = W_, D = D_;
   static_assert((TT * (TT + 1) / 2) % D == 0, "ring depth must divide a shell's positions");
@@ -84,9 +86,9 @@ template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
     case 2: return f(DenseShape<2, 4, 2, 3>{});
     case 4: return f(DenseShape<4, 2, 2, 5>{});
     case 8: return f(DenseShape<8, 2, 2, 9>{});
-    case 16: return f(DenseShape<16, 2, 1, 8>{});
-    case 28: return f(DenseShape<28, 1, 1, 14>{});
-    case 32: return f(DenseShape<32, 1, 1, 16>{});
+    case 16: return f(DenseShape<16, 2, 2, 2>{});
+    case 28: return f(DenseShape<28, 1, 2, 7>{});
+    case 32: return f(DenseShape<32, 1, 2, 4>{});
     default: return hipErrorInvalidValue;
   }
 }
@@ -832,8 +834,8 @@ template <class Sh> hipError_t batch_dense_k(const BatchParams &bp, uint32_t blo
   return hipGetLastError();
 }
 
-// the paired form exists for the shapes budgeted for two wavefronts per SIMD
-template <class Sh> constexpr bool ws_shape() { return Sh::W == 2; }
+// the paired form exists for the tile bounds up to 8 (256 sites)
+template <class Sh> constexpr bool ws_shape() { return Sh::W == 2 && Sh::TT <= 8; }
 template <class Sh> constexpr int ws_ring() { return Sh::D; }
 
 template <class Sh> const void *ws_fn(bool persistent) {
