@@ -139,9 +139,10 @@ class _Hip:
 
 def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
     """explicit scoring of `batch` uniformly random candidates per launch (seed 7) on the FCC
-    4x4x4 (256 sites) and 6x6x6 (864 sites) quinary 7-shell cells, by the sparse lane kernel
-    and the dense MFMA kernel; device-resident buffers, events on the launch stream; bond
-    counts of the two kernels compared bit for bit"""
+    4x4x4 (256 sites) and 6x6x6 (864 sites) quinary 7-shell cells, by the sparse kernels
+    (wavefront per candidate over the LDS-staged pair list; lane per candidate) and the dense
+    MFMA kernel; device-resident buffers, events on the launch stream; bond counts of the
+    three kernels compared bit for bit"""
     import ctypes as C
     import numpy as np
     import sqsgenerator_amd._lib as L
@@ -167,7 +168,8 @@ def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
         d_bo = hip.malloc(batch * S * K * K * 4)
         d_ob = hip.malloc(batch * 8)
         res, bonds = {}, {}
-        for kname, kernel in (("lane", L.KERNEL_SPARSE_LANE), ("dense", L.KERNEL_DENSE_MFMA)):
+        for kname, kernel in (("wave", L.KERNEL_SPARSE_WAVE), ("lane", L.KERNEL_SPARSE_LANE),
+                              ("dense", L.KERNEL_DENSE_MFMA)):
             def launch():
                 L.check(L.lib().sqsg_score_batch_device(job.handle, 0, d_sp, batch, d_bo, d_ob,
                                                         None, kernel, stream))
@@ -185,7 +187,8 @@ def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
             host = np.empty((batch, S, K, K), dtype=np.int32)
             hip.download(host, d_bo)
             bonds[kname] = host
-        res["bonds_identical"] = bool(np.array_equal(bonds["lane"], bonds["dense"]))
+        res["bonds_identical"] = bool(np.array_equal(bonds["wave"], bonds["dense"]) and
+                                      np.array_equal(bonds["lane"], bonds["dense"]))
         res["sites"], res["dense_shape"] = N, job.kernel_shape()
         for p in (d_sp, d_bo, d_ob):
             hip.h.hipFree(p)

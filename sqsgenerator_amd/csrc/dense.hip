@@ -431,8 +431,16 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   const bool active = gl && b < p.B;
   if (gl) {
     uint8_t *row = rows + lane * p.dense_stride;
-    for (uint32_t i = 0; i < 32 * pr.dT; ++i)
-      row[i] = (active && i < pr.N) ? p.species[b * pr.N + i] : (i < pr.N ? 0 : 0xff);
+    if ((pr.N & 3u) == 0) {  // whole words (rows of the batch stay 4-byte aligned)
+      const uint32_t *src =
+          reinterpret_cast<const uint32_t *>(p.species + (active ? b : 0) * pr.N);
+      uint32_t *dst = reinterpret_cast<uint32_t *>(row);
+      for (uint32_t w = 0; w < 8 * pr.dT; ++w)
+        dst[w] = 4 * w < pr.N ? (active ? src[w] : 0u) : 0xffffffffu;
+    } else {
+      for (uint32_t i = 0; i < 32 * pr.dT; ++i)
+        row[i] = (active && i < pr.N) ? p.species[b * pr.N + i] : (i < pr.N ? 0 : 0xff);
+    }
   }
   const unsigned long long am = __ballot(active);
   unsigned long long live = 0;

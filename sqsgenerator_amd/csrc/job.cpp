@@ -809,7 +809,12 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     const RunParams rp = make_run_params(job, SQSG_KERNEL_DENSE_MFMA, 1, &blocks);
     sp.prob = P.dev();
     sp.dense_cw = rp.dense_cw;
-    sp.dense_gen = rp.gen_group;
+    // candidates per wavefront: whole passes, fewer of them when the batch would otherwise
+    // leave most SIMDs idle (aim: two wavefronts per SIMD)
+    uint32_t gen = rp.gen_group;
+    const uint64_t want = uint64_t(8) * static_cast<uint64_t>(device_cu_count());
+    while (gen > rp.dense_cw && (uint64_t(B) + gen - 1) / gen < want) gen -= rp.dense_cw;
+    sp.dense_gen = gen;
     sp.dense_stride = rp.dense_stride;
     sp.off_row = rp.off_rows[0];
     sp.off_hist = rp.off_hist;
