@@ -317,9 +317,9 @@ def main() -> int:
         print(json.dumps({"calibration_copy_bytes": n, "launches": 3}))
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU only: candidates are independent, so N GPUs would "
+        print("bench.py measures one GPU only (candidates are independent, so N GPUs would "
               "each run disjoint RNG streams / rank ranges and merge their tie sets on the "
-              "host, but multi-GPU runs are not part of this build", file=sys.stderr)
+              "host; that merge is not built)", file=sys.stderr)
         return 2
 
     import sqsgenerator_amd._lib as L

@@ -2,7 +2,7 @@
 
 Integer work must be bit-exact (bond counts, ranks); objectives are compared bit-for-bit
 (both sides evaluate the reference's term order without contraction), which is stricter
-than the 1e-12 relative tolerance the north star allows.
+than a 1e-12 relative tolerance.
 """
 import numpy as np
 import pytest
