@@ -363,7 +363,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
         p.rec_sub[slot] = tot;
       }
     }
-    wave_sync();  // rows are rewritten by the next round
+    wave_sync();  // rows are rewritten by the next round of candidates
   }
 }
 
