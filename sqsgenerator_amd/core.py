@@ -681,19 +681,20 @@ class Job:
                                         for b in range(info.num_bounds)],
                     species_z=list(info.species_z[:K]), shell_ids=list(info.shell_ids[:S]))
 
-    def score(self, species, p: int = 0, kernel: int = L.KERNEL_AUTO):
-        """Explicit candidates (B x N packed, sorted-site order) -> bonds, objective, sro"""
+    def score(self, species, p: int = 0, kernel: int = L.KERNEL_AUTO, bonds: bool = True):
+        """Explicit candidates (B x N packed, sorted-site order) -> bonds (None unless
+        requested), objective, sro"""
         info = self.problem(p)
         N, K, S = info["num_sites"], info["num_species"], info["num_shells"]
         sp = np.ascontiguousarray(species, dtype=np.uint8).reshape(-1, N)
         B = sp.shape[0]
-        bonds = np.empty((B, S, K, K), dtype=np.int32)
+        bd = np.empty((B, S, K, K), dtype=np.int32) if bonds else None
         obj = np.empty(B)
         sro = np.empty((B, S, K, K))
         L.check(L.lib().sqsg_score_batch(self.handle, p, L.ptr(sp, L.u8p), B,
-                                         L.ptr(bonds, L.i32p), L.ptr(obj, L.f64p),
-                                         L.ptr(sro, L.f64p), kernel))
-        return bonds, obj, sro
+                                         L.ptr(bd, L.i32p) if bonds else None,
+                                         L.ptr(obj, L.f64p), L.ptr(sro, L.f64p), kernel))
+        return bd, obj, sro
 
     def set_kernel(self, kernel: int) -> None:
         L.check(L.lib().sqsg_job_set_kernel(self.handle, kernel))

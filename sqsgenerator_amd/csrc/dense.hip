@@ -45,12 +45,20 @@ __device__ __forceinline__ int onehot4(uint32_t w, uint32_t bb) {
   return static_cast<int>((~t >> 7) & 0x01010101u);
 }
 
-// GEMM1 accumulator (16 counts <= 127 of one column) -> GEMM2 B operand bytes
+// GEMM1 accumulator (16 counts <= 127 of one column) -> GEMM2 B operand bytes.  Per word
+// three byte permutes (v_perm_b32: selector bytes 0-3 take bytes of the second operand, 4-7
+// of the first, 0x0c gives zero): [y0, y1, 0, 0] and [0, 0, y2, y3], then their bytes
+// merged; the shift-or form of the same packing compiles to four operations per word.
 __device__ __forceinline__ v4i pack_counts(const v16i &y) {
   v4i b;
 #pragma unroll
-  for (int w = 0; w < 4; ++w)
-    b[w] = y[4 * w] | (y[4 * w + 1] << 8) | (y[4 * w + 2] << 16) | (y[4 * w + 3] << 24);
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t lo = __builtin_amdgcn_perm(static_cast<uint32_t>(y[4 * w + 1]),
+                                              static_cast<uint32_t>(y[4 * w]), 0x0c0c0400u);
+    const uint32_t hi = __builtin_amdgcn_perm(static_cast<uint32_t>(y[4 * w + 3]),
+                                              static_cast<uint32_t>(y[4 * w + 2]), 0x04000c0cu);
+    b[w] = static_cast<int>(__builtin_amdgcn_perm(hi, lo, 0x07060100u));
+  }
   return b;
 }
 
@@ -145,12 +153,13 @@ __device__ __forceinline__ void shell_terms(const PR &pr, uint32_t s, const uint
 }
 
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
-// bytes, padding sites 0xff); `cnt` is CW * K * K words of LDS.  Passes whose bit in `live`
-// are clear are skipped.  With `prune_key` below ~0, a pass stops after the first shell at
-// which no active candidate of it (bit set in `active`) has a partial objective key at or
-// below prune_key: every term is >= 0 (the host checks the pair weights), so partial sums
-// only grow and such candidates can never be kept.  Returns the objective of the lane's own candidate
-// (lane < npass * CW) and, when given, writes its ordered bonds and SRO cube.
+// bytes, padding sites 0xff); `cnt` is CW * K * K + 64 words of LDS.  Passes whose bit in
+// `live` are clear are skipped.  With `prune_key` below ~0, a pass stops after the first
+// shell at which no active candidate of it (bit set in `active`) has a partial objective key
+// at or below prune_key: every term is >= 0 (the host checks the pair weights), so partial
+// sums only grow and such candidates can never be kept.  Returns the objective of the
+// lane's own candidate (lane < npass * CW) and, when given, writes its ordered bonds and SRO
+// cube.
 template <int TT, int G, int D, class PR>
 __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
                                               uint32_t stride, uint32_t *cnt,
@@ -173,6 +182,7 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   const uint32_t my_pass = static_cast<uint32_t>(lane) / CW;
   const uint32_t my_cc = static_cast<uint32_t>(lane) - my_pass * CW;  // candidate in its pass
   const uint32_t cube = pr.S * KK;
+  const uint32_t trash = CW * KK + static_cast<uint32_t>(lane);  // the lane's spare word
   uint32_t keep = 0;  // accumulator registers of GEMM2 on this lane's diagonal block
 #pragma unroll
   for (uint32_t r = 0; r < 16; ++r) {
@@ -185,9 +195,9 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   auto terms_of = [&](uint32_t opass, uint32_t os) {
     if (my_pass == opass) {
       const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
-      const bool out = (bonds || sro) && cr < out_rows;
-      shell_terms(pr, os, cnt + my_cc * KK, obj, out ? bonds + cr * cube : nullptr,
-                  out ? sro + cr * cube : nullptr);
+      const bool out = cr < out_rows;
+      shell_terms(pr, os, cnt + my_cc * KK, obj, out && bonds ? bonds + cr * cube : nullptr,
+                  out && sro ? sro + cr * cube : nullptr);
     }
   };
   for (uint32_t pass = 0; pass < npass; ++pass) {
@@ -262,15 +272,17 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
         if (++J == TT) ++I, J = I;
       }
       // diagonal blocks -> counts [candidate of the pass][a][b]: register r of lane l is
-      // row m = 8(r/4) + 4h + r%4 = cK + a of column n = cK + b
+      // row m = 8(r/4) + 4h + r%4 = cK + a of column n = cK + b.  Every register is
+      // stored, the ones outside the lane's diagonal block into the lane's own word past
+      // the counts: a select per store instead of a branch
 #pragma unroll
       for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-        for (uint32_t r = 0; r < 16; ++r)
-          if ((keep >> r) & 1u) {
-            const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
-            cnt[gi * g * KK + K * m + b] = static_cast<uint32_t>(C[gi][r]);
-          }
+        for (uint32_t r = 0; r < 16; ++r) {
+          const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
+          const uint32_t at = ((keep >> r) & 1u) ? gi * g * KK + K * m + b : trash;
+          cnt[at] = static_cast<uint32_t>(C[gi][r]);
+        }
       wave_sync();
       terms_of(pass, s);
       wave_sync();  // the counts are rewritten by the next shell

@@ -578,8 +578,8 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     const uint32_t g = std::min(32 / P.K, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
-    // one shell's counts of a pass
-    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4);
+    // one shell's counts of a pass, then one spare word per lane
+    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4 + 64 * 4);
     uint32_t npass = 64 / p.dense_cw;
     // at most 20 KiB per wavefront: eight resident wavefronts per CU (two per SIMD)
     while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 20 * 1024) --npass;

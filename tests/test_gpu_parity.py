@@ -107,6 +107,28 @@ def test_score_batch_bit_exact(gpu, name, kernel):
     job.close()
 
 
+@pytest.mark.parametrize("name", SINGLE_PROBLEM)
+def test_score_dense_objective_only(gpu, name):
+    """explicit scoring of objectives and SRO cubes without the ordered bond counts
+    (a null bonds buffer): bit-exact like the full output"""
+    print(f"[score_dense_objective_only] {name}", flush=True)
+    cfg = parse_config(WORKLOADS[name]())
+    job = Job(cfg)
+    ref = oracle_inputs(cfg, job, 0)
+    rng = np.random.default_rng(20261204)
+    B = 96 if job.problem(0)["num_sites"] < 500 else 16
+    cands = random_candidates(ref["packed"], ref["bounds"], B, rng)
+    cands[1] = ref["packed"]
+    bonds, obj, sro = job.score(cands, 0, kernel=KERNELS["dense"], bonds=False)
+    assert bonds is None
+    for b in range(B):
+        rb = O.count_bonds(ref["pairs"], cands[b], ref["S"], ref["K"])
+        ro, rsro = O.compute_objective(rb, ref["pref"], ref["pw"], ref["tgt"])
+        assert obj[b].view(np.uint64) == np.float64(ro).view(np.uint64), (b, obj[b], ro)
+        assert np.array_equal(sro[b], rsro)
+    job.close()
+
+
 def test_systematic_all_results(gpu):
     """python/tests/test_core.py:45-67: pair_weights 0 -> every permutation is optimal"""
     cfg = parse_config(W.reference_test_16())
