@@ -101,13 +101,25 @@ template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
   }
 }
 
-// byte offset of upper tile (I, J) of shell s in the stream (every I <= J < T tile is
-// stored, zero tiles included), or an offset past the end (the load returns zeros)
-__device__ __forceinline__ uint32_t tile_offset(uint32_t s, uint32_t I, uint32_t J, uint32_t S,
+// Byte offset in the tile stream of upper tile (I, J), grid position q (row-major over
+// I <= J < TT), of shell s; past the end of the stream bounds-checked loads read zeros.
+//  * Tile bounds up to 8: the stream holds the kernel's whole upper grid for every shell
+//    (tiles past the problem's T tiles are zeros), so the offset is a constant plus a
+//    shell base (fewer scalar registers than per-position conditions; measured faster);
+//  * larger tile bounds: the stream holds the I <= J < T tiles; positions past the problem
+//    or the last shell get an offset past the end.  With compile-time offsets the
+//    scheduler sinks the loads of these long shells to their MFMAs (measured 35-40%
+//    slower), while the per-position conditions keep them D positions ahead.
+template <int TT>
+__device__ __forceinline__ uint32_t tile_offset(uint32_t s, int q, int I, int J, uint32_t S,
                                                 uint32_t T) {
-  if (s >= S || I >= T || J >= T) return 0x7fff0000u;
-  const uint32_t per_shell = T * (T + 1) / 2;
-  return (s * per_shell + I * T - I * (I - 1) / 2 + (J - I)) * 1024u;
+  if constexpr (TT <= 8) {
+    return (s * static_cast<uint32_t>(TT * (TT + 1) / 2) + static_cast<uint32_t>(q)) * 1024u;
+  } else {
+    const uint32_t i = static_cast<uint32_t>(I), j = static_cast<uint32_t>(J);
+    if (s >= S || i >= T || j >= T) return 0x7fff0000u;
+    return (s * (T * (T + 1) / 2) + i * T - i * (i - 1) / 2 + (j - i)) * 1024u;
+  }
 }
 
 // One shell's objective terms (optimization.h:85-100) of the candidate whose K x K counts
@@ -229,8 +241,8 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
 #pragma unroll
       for (int q = 0; q < D; ++q) {
         ring[q] = __builtin_bit_cast(
-            v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, tile_offset(0, I2, J2, S, T) + toff,
-                                                       0, 0));
+            v4i, __builtin_amdgcn_raw_buffer_load_b128(
+                     rs, tile_offset<TT>(0, q, I2, J2, S, T) + toff, 0, 0));
         if (++J2 == TT) ++I2, J2 = I2;
       }
     }
@@ -255,7 +267,8 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
         const v4i u = ring[q % D];
         ring[q % D] = __builtin_bit_cast(
             v4i, __builtin_amdgcn_raw_buffer_load_b128(
-                     rs, tile_offset(s + s2, I2, J2, S, T) + toff, 0, 0));
+                     rs, tile_offset<TT>(s + s2, s2 ? q + D - P : q + D, I2, J2, S, T) + toff,
+                     0, 0));
         if (++J2 == TT) {
           ++I2, J2 = I2;
           if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
@@ -566,8 +579,8 @@ __device__ __forceinline__ void ws_mfma_tick(const PR &pr, const uint8_t *rows, 
 #pragma unroll
     for (int q = 0; q < D; ++q) {
       ring[q] = __builtin_bit_cast(
-          v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, tile_offset(0, I2, J2, S, T) + toff,
-                                                     0, 0));
+          v4i, __builtin_amdgcn_raw_buffer_load_b128(
+                   rs, tile_offset<TT>(0, q, I2, J2, S, T) + toff, 0, 0));
       if (++J2 == TT) ++I2, J2 = I2;
     }
   }
@@ -589,8 +602,9 @@ __device__ __forceinline__ void ws_mfma_tick(const PR &pr, const uint8_t *rows, 
     }
     const v4i u = ring[q % D];
     ring[q % D] = __builtin_bit_cast(
-        v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, tile_offset(s + s2, I2, J2, S, T) + toff,
-                                                   0, 0));
+        v4i, __builtin_amdgcn_raw_buffer_load_b128(
+                 rs, tile_offset<TT>(s + s2, s2 ? q + D - P : q + D, I2, J2, S, T) + toff, 0,
+                 0));
     if (++J2 == TT) {
       ++I2, J2 = I2;
       if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;

@@ -219,20 +219,25 @@ void Problem::build_lane(uint32_t field_bits) {
 
 // Tiles in the A-operand layout of v_mfma_i32_32x32x32_i8: lane l, byte q of tile (s, I, J)
 // is U_s[32I + perm(l % 32)][32J + 16(l / 32) + q] with perm(m) = 16((m / 4) % 2) +
-// 4(m / 8) + m % 4 (the accumulator row order of the 32 x 32 result).  Every tile with
-// I <= J < T is stored (shell, then I, then J), zero tiles included, so a tile's position
-// is a function of (s, I, J) alone.
+// 4(m / 8) + m % 4 (the accumulator row order of the 32 x 32 result).  Tiles are stored
+// shell, then I, then J >= I, zero tiles included: for kernel tile bounds up to 8 the whole
+// upper grid of the bound TT (tiles past the problem's T tiles are zeros, so an offset is a
+// function of the shell and the grid position alone), for larger bounds the I <= J < T
+// tiles (the kernel checks the bounds per position).
 void Problem::build_dense() {
   if (d_dtiles.size()) return;
   dT = (N + 31) / 32;
-  const uint32_t n32 = 32 * dT;
+  const uint32_t n32 = 32 * dT, TB = dense_tile_bound(dT), TT = TB <= 8 ? TB : dT;
   std::vector<uint8_t> sh(size_t(n32) * n32, 0xff);
   for (size_t q = 0; q < pi.size(); ++q) sh[size_t(pi[q]) * n32 + pj[q]] = static_cast<uint8_t>(ps[q]);
   dtiles_nonzero = 0;
   std::vector<uint8_t> tiles;
   for (uint32_t s = 0; s < S; ++s)
-    for (uint32_t I = 0; I < dT; ++I)
-      for (uint32_t J = I; J < dT; ++J) {
+    for (uint32_t I = 0; I < TT; ++I)
+      for (uint32_t J = I; J < TT; ++J) {
+        const size_t base = tiles.size();
+        tiles.resize(base + 1024, 0);
+        if (I >= dT || J >= dT) continue;
         bool any = false;
         for (uint32_t i = 32 * I; i < 32 * I + 32 && !any; ++i)
           for (uint32_t j = 32 * J; j < 32 * J + 32; ++j)
@@ -240,8 +245,6 @@ void Problem::build_dense() {
               any = true;
               break;
             }
-        const size_t base = tiles.size();
-        tiles.resize(base + 1024, 0);
         if (!any) continue;
         ++dtiles_nonzero;
         for (uint32_t l = 0; l < 64; ++l) {
