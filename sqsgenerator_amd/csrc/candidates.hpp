@@ -94,12 +94,6 @@ __device__ void next_perm_candidate(Strided<SH> v, uint32_t lo, uint32_t w) {
   }
 }
 
-// Wave-local state of the tie-keeping filter: the L smallest distinct objectives this
-// wave has seen for the current job; its L-th bounds the job's global threshold.
-struct LevelFilter {
-  double lvl[kMaxLevels];
-  uint32_t n;
-};
 
 // Offers the objectives of the lanes in `pass` (a ballot) to the filter and lowers the
 // job's threshold when the wave holds L levels below it.  `g` is the threshold read

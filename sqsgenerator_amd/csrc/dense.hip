@@ -165,7 +165,7 @@ __device__ __forceinline__ void shell_terms(const PR &pr, uint32_t s, const uint
 }
 
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
-// bytes, padding sites 0xff); `cnt` is CW * K * K + 64 words of LDS.  Passes whose bit in
+// bytes, padding sites 0xff); `cnt` is CW * K * K + 32 words of LDS.  Passes whose bit in
 // `live` are clear are skipped.  With `prune_key` below ~0, a pass stops after the first
 // shell at which no active candidate of it (bit set in `active`) has a partial objective key
 // at or below prune_key: every term is >= 0 (the host checks the pair weights), so partial
@@ -194,7 +194,8 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   const uint32_t my_pass = static_cast<uint32_t>(lane) / CW;
   const uint32_t my_cc = static_cast<uint32_t>(lane) - my_pass * CW;  // candidate in its pass
   const uint32_t cube = pr.S * KK;
-  const uint32_t trash = CW * KK + static_cast<uint32_t>(lane);  // the lane's spare word
+  // a spare word past the counts (lanes l and l + 32 share one: its value is never read)
+  const uint32_t trash = CW * KK + (static_cast<uint32_t>(lane) & 31u);
   uint32_t keep = 0;  // accumulator registers of GEMM2 on this lane's diagonal block
 #pragma unroll
   for (uint32_t r = 0; r < 16; ++r) {
@@ -286,8 +287,8 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
       }
       // diagonal blocks -> counts [candidate of the pass][a][b]: register r of lane l is
       // row m = 8(r/4) + 4h + r%4 = cK + a of column n = cK + b.  Every register is
-      // stored, the ones outside the lane's diagonal block into the lane's own word past
-      // the counts: a select per store instead of a branch
+      // stored, the ones outside the lane's diagonal block into a spare word past the
+      // counts: a select per store instead of a branch
 #pragma unroll
       for (int gi = 0; gi < G; ++gi)
 #pragma unroll
@@ -399,7 +400,8 @@ __global__ __launch_bounds__(64, W) void run_dense_kernel(RunParams p) {
   const uint64_t per_seg = static_cast<uint64_t>(p.gen_group) * p.rounds;
   const uint64_t total = p.cand_end - p.cand_begin;
   const uint64_t nseg = (total + per_seg - 1) / per_seg;
-  LevelFilter lf;
+  // the tie filter's levels are the same on every lane: one copy per wavefront in LDS
+  LevelFilter &lf = *reinterpret_cast<LevelFilter *>(lds + p.off_levels);
   lf.n = 0;
   unsigned long long scored = 0;
   // segments are pulled from a counter: wavefronts that start late (more blocks than fit
@@ -423,8 +425,7 @@ __global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
   const uint32_t num_items = *bp.num_items;
   uint32_t last_job = 0xffffffffu, it = 0, job = 0;
   uint64_t cbeg = 0, cend = 0;
-  LevelFilter lf;
-  lf.n = 0;
+  LevelFilter *lf = nullptr;  // the job's tie filter levels, one copy per wavefront in LDS
   unsigned long long scored = 0;
   while (claim_chunk(bp, num_items, it, lane, job, cbeg, cend)) {
     // the job table is read-only during the launch: scalar loads through the constant
@@ -434,10 +435,11 @@ __global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
       if (last_job != 0xffffffffu && lane == 0 && scored)
         atomicAdd(bp.jobs[last_job].scored, scored);
       scored = 0;
-      lf.n = 0;
+      lf = reinterpret_cast<LevelFilter *>(lds + p.off_levels);
+      lf->n = 0;
       last_job = job;
     }
-    dense_segment<TT, G, D>(p, cbeg, cend, lds, lane, lf, scored);
+    dense_segment<TT, G, D>(p, cbeg, cend, lds, lane, *lf, scored);
   }
   if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
 }

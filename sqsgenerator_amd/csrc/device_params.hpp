@@ -54,6 +54,14 @@ struct DevProblem {
   uint8_t sys_rev[kMaxLocalSpecies];
 };
 
+// Wave-local state of the tie-keeping filter: the L smallest distinct objectives a
+// wavefront has seen for its current job; its L-th bounds the job's global threshold.
+// The dense kernels keep it in LDS, one copy per wavefront.
+struct LevelFilter {
+  double lvl[kMaxLevels];
+  uint32_t n;
+};
+
 struct RunParams {
   DevProblem prob[kMaxProblems];
   uint32_t nprob;
@@ -70,6 +78,7 @@ struct RunParams {
   uint32_t wave_lds;
   uint32_t off_rows[kMaxProblems];
   uint32_t off_hist, off_terms, off_obj;  // dense: off_hist holds one shell's counts
+  uint32_t off_levels;         // dense: the wavefront's tie filter levels (LevelFilter)
   uint32_t ws_pair;            // dense paired form: LDS bytes of one pair's region
   uint32_t prune;              // dense: stop a pass once no candidate can reach the kept levels
   // tie-keeping threshold and record sink

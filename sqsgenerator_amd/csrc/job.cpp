@@ -581,15 +581,18 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     const uint32_t g = std::min(32 / P.K, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
-    // one shell's counts of a pass, then one spare word per lane
-    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4 + 64 * 4);
+    // one shell's counts of a pass and 32 spare words; then the tie filter levels
+    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4 + 32 * 4);
+    const uint32_t lf_bytes = align(static_cast<uint32_t>(sizeof(LevelFilter)));
     uint32_t npass = 64 / p.dense_cw;
     // at most 20 KiB per wavefront: eight resident wavefronts per CU (two per SIMD)
-    while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes > 20 * 1024) --npass;
+    while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes + lf_bytes > 20 * 1024)
+      --npass;
     p.gen_group = npass * p.dense_cw;
     p.off_rows[0] = 0;
     p.off_hist = align(p.gen_group * p.dense_stride);
-    p.wave_lds = p.off_hist + cnt_bytes;
+    p.off_levels = p.off_hist + cnt_bytes;
+    p.wave_lds = p.off_levels + lf_bytes;
     if (p.wave_lds > 160 * 1024) throw std::out_of_range("problem does not fit the LDS budget");
     // one wavefront per block, as many as registers and LDS keep resident
     *blocks = static_cast<uint32_t>(device_cu_count()) *
