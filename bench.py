@@ -359,7 +359,9 @@ def main() -> int:
             j.set_kernel(L.KERNEL_AUTO)
             j.set_pruning(prune)
         for j in jobs:
-            j.score(j.problem(0)["species_packed"][None, :])  # inputs resident in HBM
+            # inputs resident in HBM: pair lists, parameter cubes and the dense tile stream
+            # (explicit scoring through the dense kernel builds and uploads it)
+            j.score(j.problem(0)["species_packed"][None, :], kernel=L.KERNEL_DENSE_MFMA)
         return jobs
 
     if args.warmup:
