@@ -282,9 +282,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--candidates-per-step", type=int, default=1 << 21,
+    ap.add_argument("--candidates-per-step", type=int, default=1 << 23,
                     help="candidates of each step (a launch of the plain kernel, or one "
-                         "persistent step over the batched jobs)")
+                         "persistent step over the batched jobs); a persistent step ends "
+                         "with its slowest wavefront's last 60-candidate segment, so longer "
+                         "steps waste less of the launch")
     ap.add_argument("--batch-jobs", type=int, default=4)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--serve-rate", type=float, default=30.0)
