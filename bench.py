@@ -373,7 +373,7 @@ def main() -> int:
             j.close()
     jobs = batch_jobs(args.steps)
     info = jobs[0].problem(0)
-    shape = jobs[0].kernel_shape()  # e.g. dense<8,2,2,9>
+    shape = jobs[0].kernel_shape()  # e.g. dense<8,2,2,3>
     kernel_name = ("batch_dense_kernel" + shape[len("dense"):] if shape.startswith("dense")
                    else shape)
     L.check(L.lib().sqsg_device_synchronize())

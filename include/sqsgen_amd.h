@@ -233,7 +233,7 @@ SQSG_API int sqsg_batch_run(sqsg_job **jobs, uint32_t num_jobs, int use_graph,
  * (written to *chosen, also stored in the job). */
 SQSG_API int sqsg_autotune(sqsg_job *job, int *chosen, double *cand_per_s);
 /* Launch shape of the kernel a run of `job` uses (no reference counterpart: introspection
- * for profiling and reports), e.g. "dense<8,2,2,9>" (tile bound, tile groups, wavefronts
+ * for profiling and reports), e.g. "dense<8,2,2,3>" (tile bound, tile groups, wavefronts
  * per SIMD, ring depth), "paired<8,2,9>", "lane<5>" or "wave"; NUL-terminated, truncated
  * to `len` bytes. */
 SQSG_API int sqsg_job_kernel_shape(sqsg_job *job, char *buf, size_t len);

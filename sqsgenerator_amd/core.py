@@ -741,7 +741,7 @@ class Job:
         return SqsResultPack._from_job(self)
 
     def kernel_shape(self) -> str:
-        """launch shape of the kernel a run uses, e.g. "dense<8,2,2,9>" """
+        """launch shape of the kernel a run uses, e.g. "dense<8,2,2,3>" """
         buf = C.create_string_buffer(128)
         L.check(L.lib().sqsg_job_kernel_shape(self.handle, buf, len(buf)))
         return buf.value.decode()

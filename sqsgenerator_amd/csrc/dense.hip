@@ -84,16 +84,20 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T &constant_r
 // the large tile bounds the one-hot tiles take most of the registers, so their rings are
 // short (measured: a second wavefront per SIMD beats a deeper ring by 33-58% at 500-1024
 // sites).
-template <int TT_, int G_, int W_, int D_> struct DenseShape {
-  static constexpr int TT = TT_, G = G_, W = W_, D = D_;
-  static_assert((TT * (TT + 1) / 2) % D == 0, "ring depth must divide a shell's positions");
+// DS is the ring depth of the explicit-scoring and paired kernels, which run with one
+// resident wavefront per SIMD where a deeper ring hides the tile loads; the persistent and
+// single-job kernels share their SIMD between two wavefronts and keep the registers instead.
+template <int TT_, int G_, int W_, int D_, int DS_ = D_> struct DenseShape {
+  static constexpr int TT = TT_, G = G_, W = W_, D = D_, DS = DS_;
+  static_assert((TT * (TT + 1) / 2) % D == 0 && (TT * (TT + 1) / 2) % DS == 0,
+                "ring depth must divide a shell's positions");
 };
 template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
   switch (tile_bound) {
     case 1: return f(DenseShape<1, 4, 2, 1>{});
     case 2: return f(DenseShape<2, 4, 2, 3>{});
     case 4: return f(DenseShape<4, 2, 2, 5>{});
-    case 8: return f(DenseShape<8, 2, 2, 9>{});
+    case 8: return f(DenseShape<8, 2, 2, 3, 9>{});
     case 16: return f(DenseShape<16, 2, 2, 2>{});
     case 28: return f(DenseShape<28, 1, 2, 7>{});
     case 32: return f(DenseShape<32, 1, 2, 4>{});
@@ -872,7 +876,7 @@ template <class Sh> hipError_t batch_dense_k(const BatchParams &bp, uint32_t blo
 
 // the paired form exists for the tile bounds up to 8 (256 sites)
 template <class Sh> constexpr bool ws_shape() { return Sh::W == 2 && Sh::TT <= 8; }
-template <class Sh> constexpr int ws_ring() { return Sh::D; }
+template <class Sh> constexpr int ws_ring() { return Sh::DS; }
 
 template <class Sh> const void *ws_fn(bool persistent) {
   return persistent ? reinterpret_cast<const void *>(&batch_dense_ws_kernel<Sh::TT, Sh::G, ws_ring<Sh>()>)
@@ -880,11 +884,11 @@ template <class Sh> const void *ws_fn(bool persistent) {
 }
 
 template <class Sh> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s) {
-  const void *fn = reinterpret_cast<const void *>(&score_dense_kernel<Sh::TT, Sh::G, Sh::D>);
+  const void *fn = reinterpret_cast<const void *>(&score_dense_kernel<Sh::TT, Sh::G, Sh::DS>);
   const hipError_t e = dense_lds(fn, p.wave_lds);
   if (e != hipSuccess) return e;
   const uint32_t blocks = static_cast<uint32_t>((p.B + p.dense_gen - 1) / p.dense_gen);
-  hipLaunchKernelGGL((score_dense_kernel<Sh::TT, Sh::G, Sh::D>), dim3(blocks), dim3(64),
+  hipLaunchKernelGGL((score_dense_kernel<Sh::TT, Sh::G, Sh::DS>), dim3(blocks), dim3(64),
                      p.wave_lds, s, p);
   return hipGetLastError();
 }

@@ -190,7 +190,7 @@ def test_kernel_shape_follows_the_applicability_rules():
     jobs of at most 256 sites; the lane kernel is sized by K"""
     from sqsgenerator_amd.core import Job
     job = Job(parse_config(W.quinary_256(iterations=1000)))
-    assert job.kernel_shape() == "dense<8,2,2,9>"
+    assert job.kernel_shape() == "dense<8,2,2,3>"
     job.set_kernel(4)
     assert job.kernel_shape() == "paired<8,2,9>"
     job.set_kernel(2)
