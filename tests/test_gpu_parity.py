@@ -540,7 +540,9 @@ def test_pruned_batch_and_systematic(gpu):
 
 def _cell(name):
     """cells that exercise the remaining dense shapes: tile bound 16 (500 sites), 32 (1024
-    sites, the largest) and 16 species (the dense kernel's maximum, beyond the lane kernel)"""
+    sites, the largest), 16 species (the dense kernel's maximum, beyond the lane kernel),
+    and fewer 32-site tiles than the tile bound (96 sites: 3 tiles under bound 4; 192 sites:
+    6 under bound 8), where the stream's grid positions past the problem hold zero tiles"""
     metals16 = ["Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Zr", "Nb", "Mo", "Ru",
                 "Rh", "Pd", "Ag"]
     if name == "fcc500_quinary":
@@ -552,11 +554,18 @@ def _cell(name):
     if name == "fcc256_16species":
         comp = {m: 16 for m in metals16}
         return W.fcc(4.05, (4, 4, 4), comp, 3, seed=73, thread_config=1)
+    if name == "fcc96_quaternary":
+        comp = {"Fe": 24, "Co": 24, "Ni": 24, "Cr": 24}
+        return W.fcc(4.05, (2, 3, 4), comp, 4, seed=74, thread_config=1)
+    if name == "fcc192_quinary":
+        comp = {"Fe": 39, "Co": 39, "Ni": 38, "Cr": 38, "Mn": 38}
+        return W.fcc(4.05, (4, 4, 3), comp, 5, seed=75, thread_config=1)
     raise KeyError(name)
 
 
 @pytest.mark.parametrize("name,ref_kernel", [("fcc500_quinary", 2), ("fcc1024_ternary", 2),
-                                             ("fcc256_16species", 1)])
+                                             ("fcc256_16species", 1), ("fcc96_quaternary", 2),
+                                             ("fcc192_quinary", 2)])
 def test_dense_shapes_match_sparse(gpu, name, ref_kernel):
     """explicit scoring of random candidates (bit-exact bonds, objectives and SRO against the
     sparse kernel, bonds against the oracle on a sample) and a short MC run whose pack equals
