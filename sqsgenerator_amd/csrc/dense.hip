@@ -168,6 +168,58 @@ __device__ __forceinline__ void shell_terms(const PR &pr, uint32_t s, const uint
   }
 }
 
+// Search kernels of tile bounds up to 8 read the objective terms from value tables: +1%
+// at 256 sites, while at 500-1024 sites the tables compete with the tile stream for L2
+// (-4% measured at 1024 sites), so the larger bounds keep the arithmetic.
+template <int TT> constexpr bool kDenseTermTables = TT <= 8;
+
+// The same terms from the job's value tables (search kernels, no SRO or bond output): term
+// t's value is otab[shell start + t * stride + c] for its bond count c = c_ab + c_ba, the
+// host having applied the operations above to every count; the loads of a shell are
+// independent and only the additions keep the reference order.
+template <int KT, class PR>
+__device__ __forceinline__ void shell_terms_table_k(const PR &pr, uint32_t s, const uint32_t *cc,
+                                                    double &obj, uint32_t Kr) {
+  const uint32_t K = KT > 0 ? static_cast<uint32_t>(KT) : Kr;
+  const uint32_t at = constant_ref(pr.otab_shell, 2 * s);
+  const uint32_t stride = constant_ref(pr.otab_shell, 2 * s + 1);
+  const double *tab = pr.otab + at;
+  if constexpr (KT > 0) {
+    constexpr int NT = KT * (KT + 1) / 2;
+    double v[NT];
+    int t = 0;
+#pragma unroll
+    for (int xi = 0; xi < KT; ++xi)
+#pragma unroll
+      for (int eta = xi; eta < KT; ++eta, ++t) {
+        const uint32_t c = cc[xi * KT + eta] + (xi == eta ? 0u : cc[eta * KT + xi]);
+        v[t] = tab[static_cast<uint32_t>(t) * stride + c];
+      }
+#pragma unroll
+    for (int u = 0; u < NT; ++u) obj += v[u];
+  } else {
+    uint32_t t = 0;
+    for (uint32_t xi = 0; xi < K; ++xi)
+      for (uint32_t eta = xi; eta < K; ++eta, ++t) {
+        const uint32_t c = cc[xi * K + eta] + (xi == eta ? 0u : cc[eta * K + xi]);
+        obj += tab[t * stride + c];
+      }
+  }
+}
+
+template <class PR>
+__device__ __forceinline__ void shell_terms_table(const PR &pr, uint32_t s, const uint32_t *cc,
+                                                  double &obj) {
+  switch (pr.K) {
+    case 2: return shell_terms_table_k<2>(pr, s, cc, obj, 2);
+    case 3: return shell_terms_table_k<3>(pr, s, cc, obj, 3);
+    case 4: return shell_terms_table_k<4>(pr, s, cc, obj, 4);
+    case 5: return shell_terms_table_k<5>(pr, s, cc, obj, 5);
+    case 6: return shell_terms_table_k<6>(pr, s, cc, obj, 6);
+    default: return shell_terms_table_k<0>(pr, s, cc, obj, pr.K);
+  }
+}
+
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
 // bytes, padding sites 0xff); `cnt` is CW * K * K + 32 words of LDS.  Passes whose bit in
 // `live` are clear are skipped.  With `prune_key` below ~0, a pass stops after the first
@@ -175,8 +227,8 @@ __device__ __forceinline__ void shell_terms(const PR &pr, uint32_t s, const uint
 // at or below prune_key: every term is >= 0 (the host checks the pair weights), so partial
 // sums only grow and such candidates can never be kept.  Returns the objective of the
 // lane's own candidate (lane < npass * CW) and, when given, writes its ordered bonds and SRO
-// cube.
-template <int TT, int G, int D, class PR>
+// cube; TABLE: the objective from the job's term tables (no bonds or SRO output).
+template <int TT, int G, int D, bool TABLE, class PR>
 __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
                                               uint32_t stride, uint32_t *cnt,
                                               uint32_t CW, uint32_t npass,
@@ -211,10 +263,14 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   // reference order
   auto terms_of = [&](uint32_t opass, uint32_t os) {
     if (my_pass == opass) {
-      const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
-      const bool out = cr < out_rows;
-      shell_terms(pr, os, cnt + my_cc * KK, obj, out && bonds ? bonds + cr * cube : nullptr,
-                  out && sro ? sro + cr * cube : nullptr);
+      if constexpr (TABLE) {
+        shell_terms_table(pr, os, cnt + my_cc * KK, obj);
+      } else {
+        const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
+        const bool out = cr < out_rows;
+        shell_terms(pr, os, cnt + my_cc * KK, obj, out && bonds ? bonds + cr * cube : nullptr,
+                    out && sro ? sro + cr * cube : nullptr);
+      }
     }
   };
   for (uint32_t pass = 0; pass < npass; ++pass) {
@@ -372,8 +428,8 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
       prune_key = g0 < l0 ? g0 : l0;
     }
     const double tot =
-        dense_score<TT, G, D>(pr, rows, stride, cnt, CW, npass, live, lane, nullptr, nullptr,
-                              0u, am, prune_key);
+        dense_score<TT, G, D, kDenseTermTables<TT>>(pr, rows, stride, cnt, CW, npass, live,
+                                                    lane, nullptr, nullptr, 0u, am, prune_key);
     scored += static_cast<unsigned long long>(__popcll(am));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -480,9 +536,9 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   wave_sync();
   const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * GEN;
   const uint32_t out_rows = static_cast<uint32_t>(p.B - b0 < GEN ? p.B - b0 : GEN);
-  const double obj = dense_score<TT, G, D>(pr, rows, p.dense_stride, cnt, CW, npass, live,
-                                           lane, p.bonds ? p.bonds + b0 * nb : nullptr,
-                                           p.sro ? p.sro + b0 * nb : nullptr, out_rows);
+  const double obj = dense_score<TT, G, D, false>(
+      pr, rows, p.dense_stride, cnt, CW, npass, live, lane,
+      p.bonds ? p.bonds + b0 * nb : nullptr, p.sro ? p.sro + b0 * nb : nullptr, out_rows);
   if (active) p.objective[b] = obj;
 }
 

@@ -34,6 +34,10 @@ struct DevProblem {
   // row tile, column tile) order
   const uint32_t *dtiles;     // 256 words per tile
   const uint8_t *species0D;   // sorted configuration, 32 * dT bytes, padding sites 0xff
+  // objective term values by bond count (dense search kernels): shell s starts at
+  // otab[otab_shell[2s]], term t of it at + t * otab_shell[2s + 1] (P_s + 1 counts)
+  const double *otab;
+  const uint32_t *otab_shell;
   uint32_t dT;                // 32-site tiles
   uint32_t dtiles_n;
   // lane-per-candidate layout: per shell s, groups [i | count << 16] followed by `count`
