@@ -24,8 +24,15 @@ template <uint32_t SH> struct Strided {
 
 // PR is DevProblem, possibly qualified with the constant address space (a job table entry
 // read with scalar loads)
+// (r * n) >> 32 over 64 bits is the high word of the 32 x 32 product, one v_mul_hi_u32.
+// Written in C the product stays 64-bit in the compiler's view, and loop strength
+// reduction then carries the shuffle's counter as a 64-bit value with an extra
+// quarter-rate v_mad_u64_u32 per step; the one-instruction asm (a plain VALU operation,
+// no memory access) keeps it a 32-bit multiply.
 __device__ __forceinline__ uint32_t bounded(uint32_t r, uint32_t n) {  // shuffle.h:28-32
-  return static_cast<uint32_t>((static_cast<uint64_t>(r) * n) >> 32);
+  uint32_t q;
+  asm("v_mul_hi_u32 %0, %1, %2" : "=v"(q) : "v"(r), "v"(n));
+  return q;
 }
 
 // The draw of step i - 1 is computed between the reads and the writes of step i, so its

@@ -721,7 +721,7 @@ template <class PR>
 __device__ __forceinline__ void fy_steps(FyState &f, const PR &pr, uint8_t *row, uint32_t n) {
   while (n > 0 && f.w < pr.nbounds) {
     if (f.i > 1) {
-      const uint32_t q = static_cast<uint32_t>((static_cast<uint64_t>(draw32(f.st)) * f.i) >> 32);
+      const uint32_t q = bounded(draw32(f.st), f.i);
       const uint8_t tmp = row[f.lo + f.i - 1];
       row[f.lo + f.i - 1] = row[f.lo + q];
       row[f.lo + q] = tmp;
