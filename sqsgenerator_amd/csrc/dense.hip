@@ -126,6 +126,20 @@ __device__ __forceinline__ uint32_t tile_offset(uint32_t s, int q, int I, int J,
   }
 }
 
+// Loads one lane's 16 bytes of the tile at stream offset `off` (uniform).  Tile bounds up
+// to 8: the offset goes in the instruction's scalar operand and the lane's part in the
+// vector operand, so a load costs no vector instruction; the scalar operand is outside the
+// buffer's range check, and every offset of this layout is inside the stream (a zero shell
+// follows the last one for the prefetch past it).  Larger bounds: the whole offset is
+// range-checked, and the offsets past the problem read zeros.
+template <int TT>
+__device__ __forceinline__ v4i tile_load(__amdgpu_buffer_rsrc_t rs, uint32_t toff, uint32_t off) {
+  if constexpr (TT <= 8)
+    return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, toff, off, 0));
+  else
+    return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, off + toff, 0, 0));
+}
+
 // One shell's objective terms (optimization.h:85-100) of the candidate whose K x K counts
 // are at cc (LDS), added to obj in the reference order (xi, then eta >= xi); bonds and SRO
 // cubes [S][K][K] are written when given.  The term parameters are uniform across the
@@ -301,9 +315,7 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
       int I2 = 0, J2 = 0;
 #pragma unroll
       for (int q = 0; q < D; ++q) {
-        ring[q] = __builtin_bit_cast(
-            v4i, __builtin_amdgcn_raw_buffer_load_b128(
-                     rs, tile_offset<TT>(0, q, I2, J2, S, T) + toff, 0, 0));
+        ring[q] = tile_load<TT>(rs, toff, tile_offset<TT>(0, q, I2, J2, S, T));
         if (++J2 == TT) ++I2, J2 = I2;
       }
     }
@@ -326,10 +338,8 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
           for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
         }
         const v4i u = ring[q % D];
-        ring[q % D] = __builtin_bit_cast(
-            v4i, __builtin_amdgcn_raw_buffer_load_b128(
-                     rs, tile_offset<TT>(s + s2, s2 ? q + D - P : q + D, I2, J2, S, T) + toff,
-                     0, 0));
+        ring[q % D] = tile_load<TT>(
+            rs, toff, tile_offset<TT>(s + s2, s2 ? q + D - P : q + D, I2, J2, S, T));
         if (++J2 == TT) {
           ++I2, J2 = I2;
           if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
@@ -640,9 +650,7 @@ __device__ __forceinline__ void ws_mfma_tick(const PR &pr, const uint8_t *rows, 
     int I2 = 0, J2 = 0;
 #pragma unroll
     for (int q = 0; q < D; ++q) {
-      ring[q] = __builtin_bit_cast(
-          v4i, __builtin_amdgcn_raw_buffer_load_b128(
-                   rs, tile_offset<TT>(0, q, I2, J2, S, T) + toff, 0, 0));
+      ring[q] = tile_load<TT>(rs, toff, tile_offset<TT>(0, q, I2, J2, S, T));
       if (++J2 == TT) ++I2, J2 = I2;
     }
   }
@@ -663,10 +671,8 @@ __device__ __forceinline__ void ws_mfma_tick(const PR &pr, const uint8_t *rows, 
       for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
     }
     const v4i u = ring[q % D];
-    ring[q % D] = __builtin_bit_cast(
-        v4i, __builtin_amdgcn_raw_buffer_load_b128(
-                 rs, tile_offset<TT>(s + s2, s2 ? q + D - P : q + D, I2, J2, S, T) + toff, 0,
-                 0));
+    ring[q % D] =
+        tile_load<TT>(rs, toff, tile_offset<TT>(s + s2, s2 ? q + D - P : q + D, I2, J2, S, T));
     if (++J2 == TT) {
       ++I2, J2 = I2;
       if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;

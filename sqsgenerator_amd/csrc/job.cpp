@@ -256,6 +256,9 @@ void Problem::build_dense() {
           }
         }
       }
+  // bounds up to 8: one zero shell after the last, so that the kernel's prefetch past the
+  // last shell stays inside the stream (those loads take their offset unchecked)
+  if (TB <= 8) tiles.resize(tiles.size() + size_t(TT) * (TT + 1) / 2 * 1024, 0);
   dtiles_n = static_cast<uint32_t>(tiles.size() / 1024);
   std::vector<uint8_t> sp(n32, 0xff);
   std::copy(species_packed.begin(), species_packed.end(), sp.begin());
