@@ -22,9 +22,9 @@
 //
 // The MFMA part of a shell is straight-line code over the TT x TT upper tile grid (TT is
 // the kernel's tile bound, T <= TT the problem's tile count): every tile load is issued a
-// fixed number of grid positions ahead into a register ring and every MFMA runs; tiles
-// past the problem's T x T grid read zeros (bounds-checked buffer loads) and zero tiles of
-// the grid are stored as zeros, so they add nothing.  Without branches the scheduler can
+// fixed number of grid positions ahead into a register ring and every MFMA runs; the tile
+// stream holds the whole TT x TT upper grid, tiles past the problem's T x T grid and zero
+// tiles of it stored as zeros, so they add nothing.  Without branches the scheduler can
 // overlap the row-end packing (accumulator reads, byte packing) with the next row's MFMAs.
 #include <hip/hip_runtime.h>
 
@@ -105,39 +105,24 @@ template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
   }
 }
 
-// Byte offset in the tile stream of upper tile (I, J), grid position q (row-major over
-// I <= J < TT), of shell s; past the end of the stream bounds-checked loads read zeros.
-//  * Tile bounds up to 8: the stream holds the kernel's whole upper grid for every shell
-//    (tiles past the problem's T tiles are zeros), so the offset is a constant plus a
-//    shell base (fewer scalar registers than per-position conditions; measured faster);
-//  * larger tile bounds: the stream holds the I <= J < T tiles; positions past the problem
-//    or the last shell get an offset past the end.  With compile-time offsets the
-//    scheduler sinks the loads of these long shells to their MFMAs (measured 35-40%
-//    slower), while the per-position conditions keep them D positions ahead.
+// Byte offset in the tile stream of grid position q (row-major over the upper tiles
+// I <= J < TT) of shell s.  The stream holds the kernel's whole upper grid for every shell
+// (tiles past the problem's T tiles are zeros) and one zero shell after the last, so the
+// offset is a constant plus a shell base and every offset the kernels form, the prefetch
+// D positions past the last shell's end included, lies inside the stream.  Measured faster
+// than a stream of the problem's own T tiles with per-position range conditions at every
+// tile bound (+4% at 16 tiles, +7% at 32).
 template <int TT>
-__device__ __forceinline__ uint32_t tile_offset(uint32_t s, int q, int I, int J, uint32_t S,
-                                                uint32_t T) {
-  if constexpr (TT <= 8) {
-    return (s * static_cast<uint32_t>(TT * (TT + 1) / 2) + static_cast<uint32_t>(q)) * 1024u;
-  } else {
-    const uint32_t i = static_cast<uint32_t>(I), j = static_cast<uint32_t>(J);
-    if (s >= S || i >= T || j >= T) return 0x7fff0000u;
-    return (s * (T * (T + 1) / 2) + i * T - i * (i - 1) / 2 + (j - i)) * 1024u;
-  }
+__device__ __forceinline__ uint32_t tile_offset(uint32_t s, int q) {
+  return (s * static_cast<uint32_t>(TT * (TT + 1) / 2) + static_cast<uint32_t>(q)) * 1024u;
 }
 
-// Loads one lane's 16 bytes of the tile at stream offset `off` (uniform).  Tile bounds up
-// to 8: the offset goes in the instruction's scalar operand and the lane's part in the
-// vector operand, so a load costs no vector instruction; the scalar operand is outside the
-// buffer's range check, and every offset of this layout is inside the stream (a zero shell
-// follows the last one for the prefetch past it).  Larger bounds: the whole offset is
-// range-checked, and the offsets past the problem read zeros.
-template <int TT>
+// Loads one lane's 16 bytes of the tile at stream offset `off` (uniform): the offset goes
+// in the instruction's scalar operand and the lane's part in the vector operand, so a load
+// costs no vector instruction.  The scalar operand is outside the buffer's range check;
+// tile_offset keeps it inside the stream.
 __device__ __forceinline__ v4i tile_load(__amdgpu_buffer_rsrc_t rs, uint32_t toff, uint32_t off) {
-  if constexpr (TT <= 8)
-    return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, toff, off, 0));
-  else
-    return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, off + toff, 0, 0));
+  return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, toff, off, 0));
 }
 
 // One shell's objective terms (optimization.h:85-100) of the candidate whose K x K counts
@@ -311,26 +296,13 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
     // Grid positions are stepped incrementally so that every index folds to a constant
     // once the loops are unrolled.
     v4i ring[D];
-    {
-      int I2 = 0, J2 = 0;
 #pragma unroll
-      for (int q = 0; q < D; ++q) {
-        ring[q] = tile_load<TT>(rs, toff, tile_offset<TT>(0, q, I2, J2, S, T));
-        if (++J2 == TT) ++I2, J2 = I2;
-      }
-    }
+    for (int q = 0; q < D; ++q) ring[q] = tile_load(rs, toff, tile_offset<TT>(0, q));
     for (uint32_t s = 0; s < S; ++s) {
       v16i C[G], Y[G];
 #pragma unroll
       for (int gi = 0; gi < G; ++gi) C[gi] = v16i{};
-      int I = 0, J = 0;             // current grid position
-      int I2 = 0, J2 = 0, s2 = 0;   // the position D ahead (s2 = 1: next shell)
-#pragma unroll
-      for (int k = 0; k < D; ++k)
-        if (++J2 == TT) {
-          ++I2, J2 = I2;
-          if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
-        }
+      int I = 0, J = 0;  // current grid position
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         if (J == I) {
@@ -338,12 +310,9 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
           for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
         }
         const v4i u = ring[q % D];
-        ring[q % D] = tile_load<TT>(
-            rs, toff, tile_offset<TT>(s + s2, s2 ? q + D - P : q + D, I2, J2, S, T));
-        if (++J2 == TT) {
-          ++I2, J2 = I2;
-          if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
-        }
+        // the position D ahead, in the next shell past this one's end
+        ring[q % D] = q + D < P ? tile_load(rs, toff, tile_offset<TT>(s, q + D))
+                                : tile_load(rs, toff, tile_offset<TT>(s + 1, q + D - P));
 #pragma unroll
         for (int gi = 0; gi < G; ++gi)
           Y[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(u, X[gi][J], Y[gi], 0, 0, 0);
@@ -647,23 +616,13 @@ __device__ __forceinline__ void ws_mfma_tick(const PR &pr, const uint8_t *rows, 
         X[gi][J] = x;
       }
     }
-    int I2 = 0, J2 = 0;
 #pragma unroll
-    for (int q = 0; q < D; ++q) {
-      ring[q] = tile_load<TT>(rs, toff, tile_offset<TT>(0, q, I2, J2, S, T));
-      if (++J2 == TT) ++I2, J2 = I2;
-    }
+    for (int q = 0; q < D; ++q) ring[q] = tile_load(rs, toff, tile_offset<TT>(0, q));
   }
   v16i C[G], Y[G];
 #pragma unroll
   for (int gi = 0; gi < G; ++gi) C[gi] = v16i{};
-  int I = 0, J = 0, I2 = 0, J2 = 0, s2 = 0;
-#pragma unroll
-  for (int k = 0; k < D; ++k)
-    if (++J2 == TT) {
-      ++I2, J2 = I2;
-      if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
-    }
+  int I = 0, J = 0;
 #pragma unroll
   for (int q = 0; q < P; ++q) {
     if (J == I) {
@@ -671,12 +630,8 @@ __device__ __forceinline__ void ws_mfma_tick(const PR &pr, const uint8_t *rows, 
       for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
     }
     const v4i u = ring[q % D];
-    ring[q % D] =
-        tile_load<TT>(rs, toff, tile_offset<TT>(s + s2, s2 ? q + D - P : q + D, I2, J2, S, T));
-    if (++J2 == TT) {
-      ++I2, J2 = I2;
-      if (I2 == TT) I2 = 0, J2 = 0, s2 = 1;
-    }
+    ring[q % D] = q + D < P ? tile_load(rs, toff, tile_offset<TT>(s, q + D))
+                            : tile_load(rs, toff, tile_offset<TT>(s + 1, q + D - P));
 #pragma unroll
     for (int gi = 0; gi < G; ++gi)
       Y[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(u, X[gi][J], Y[gi], 0, 0, 0);

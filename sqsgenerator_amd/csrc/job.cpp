@@ -227,7 +227,7 @@ void Problem::build_lane(uint32_t field_bits) {
 void Problem::build_dense() {
   if (d_dtiles.size()) return;
   dT = (N + 31) / 32;
-  const uint32_t n32 = 32 * dT, TB = dense_tile_bound(dT), TT = TB <= 8 ? TB : dT;
+  const uint32_t n32 = 32 * dT, TT = dense_tile_bound(dT);
   std::vector<uint8_t> sh(size_t(n32) * n32, 0xff);
   for (size_t q = 0; q < pi.size(); ++q) sh[size_t(pi[q]) * n32 + pj[q]] = static_cast<uint8_t>(ps[q]);
   dtiles_nonzero = 0;
@@ -256,9 +256,9 @@ void Problem::build_dense() {
           }
         }
       }
-  // bounds up to 8: one zero shell after the last, so that the kernel's prefetch past the
-  // last shell stays inside the stream (those loads take their offset unchecked)
-  if (TB <= 8) tiles.resize(tiles.size() + size_t(TT) * (TT + 1) / 2 * 1024, 0);
+  // one zero shell after the last, so that the kernels' prefetch past the last shell stays
+  // inside the stream (those loads take their offset unchecked)
+  tiles.resize(tiles.size() + size_t(TT) * (TT + 1) / 2 * 1024, 0);
   dtiles_n = static_cast<uint32_t>(tiles.size() / 1024);
   std::vector<uint8_t> sp(n32, 0xff);
   std::copy(species_packed.begin(), species_packed.end(), sp.begin());
