@@ -84,6 +84,11 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T &constant_r
 // the large tile bounds the one-hot tiles take most of the registers, so their rings are
 // short (measured: a second wavefront per SIMD beats a deeper ring by 33-58% at 500-1024
 // sites).
+// Every grid position of the bound costs its MFMAs whatever the problem's tile count, so the
+// bounds are spaced closely where cells are common (measured against the next larger bound:
+// +26% at 192 sites with 6, +53% at 384 with 12, +27% at 432 with 14, +78% at 600 with 20,
+// +27% at 768 with 24, +7% at 864 with 27; a bound of 3 was 35% slower than 4 at 96 sites,
+// its four tile groups spill).
 // DS is the ring depth of the explicit-scoring and paired kernels, which run with one
 // resident wavefront per SIMD where a deeper ring hides the tile loads; the persistent and
 // single-job kernels share their SIMD between two wavefronts and keep the registers instead.
@@ -97,9 +102,14 @@ template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
     case 1: return f(DenseShape<1, 4, 2, 1>{});
     case 2: return f(DenseShape<2, 4, 2, 3>{});
     case 4: return f(DenseShape<4, 2, 2, 5>{});
+    case 6: return f(DenseShape<6, 2, 2, 3, 7>{});
     case 8: return f(DenseShape<8, 2, 2, 3, 9>{});
+    case 12: return f(DenseShape<12, 2, 2, 3>{});
+    case 14: return f(DenseShape<14, 2, 2, 3>{});
     case 16: return f(DenseShape<16, 2, 2, 2>{});
-    case 28: return f(DenseShape<28, 1, 2, 7>{});
+    case 20: return f(DenseShape<20, 1, 2, 7>{});
+    case 24: return f(DenseShape<24, 1, 2, 6>{});
+    case 27: return f(DenseShape<27, 1, 2, 7>{});
     case 32: return f(DenseShape<32, 1, 2, 4>{});
     default: return hipErrorInvalidValue;
   }
@@ -913,7 +923,7 @@ template <class Sh> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s
 }  // namespace
 
 uint32_t dense_tile_bound(uint32_t tiles) {
-  for (uint32_t tt : {1u, 2u, 4u, 8u, 16u, 28u, 32u})
+  for (uint32_t tt : {1u, 2u, 4u, 6u, 8u, 12u, 14u, 16u, 20u, 24u, 27u, 32u})
     if (tiles <= tt) return tt;
   return 0;
 }

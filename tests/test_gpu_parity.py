@@ -539,10 +539,11 @@ def test_pruned_batch_and_systematic(gpu):
 
 
 def _cell(name):
-    """cells that exercise the remaining dense shapes: tile bound 16 (500 sites), 32 (1024
-    sites, the largest), 16 species (the dense kernel's maximum, beyond the lane kernel),
-    and fewer 32-site tiles than the tile bound (96 sites: 3 tiles under bound 4; 192 sites:
-    6 under bound 8), where the stream's grid positions past the problem hold zero tiles"""
+    """cells that exercise the remaining dense shapes: tile bounds 6 (192 sites), 12 (384),
+    14 (432 sites: 13.5 tiles, a partial last tile), 16 (500), 20 (600 sites: 19 tiles
+    under bound 20, the stream's grid positions past the problem hold zero tiles), 24 (768)
+    and 32 (1024 sites, the largest), 16 species (the dense kernel's maximum, beyond the
+    lane kernel) and 96 sites (3 tiles under bound 4)"""
     metals16 = ["Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Zr", "Nb", "Mo", "Ru",
                 "Rh", "Pd", "Ag"]
     if name == "fcc500_quinary":
@@ -560,12 +561,26 @@ def _cell(name):
     if name == "fcc192_quinary":
         comp = {"Fe": 39, "Co": 39, "Ni": 38, "Cr": 38, "Mn": 38}
         return W.fcc(4.05, (4, 4, 3), comp, 5, seed=75, thread_config=1)
+    if name == "fcc384_quaternary":
+        comp = {"Fe": 96, "Co": 96, "Ni": 96, "Cr": 96}
+        return W.fcc(4.05, (4, 4, 6), comp, 4, seed=76, thread_config=1)
+    if name == "fcc432_ternary":
+        comp = {"Fe": 144, "Co": 144, "Ni": 144}
+        return W.fcc(4.05, (3, 4, 9), comp, 5, seed=77, thread_config=1)
+    if name == "fcc600_quinary":
+        comp = {"Fe": 120, "Co": 120, "Ni": 120, "Cr": 120, "Mn": 120}
+        return W.fcc(4.05, (5, 5, 6), comp, 4, seed=78, thread_config=1)
+    if name == "fcc768_ternary":
+        comp = {"Fe": 256, "Co": 256, "Ni": 256}
+        return W.fcc(4.05, (4, 8, 6), comp, 3, seed=79, thread_config=1)
     raise KeyError(name)
 
 
 @pytest.mark.parametrize("name,ref_kernel", [("fcc500_quinary", 2), ("fcc1024_ternary", 2),
                                              ("fcc256_16species", 1), ("fcc96_quaternary", 2),
-                                             ("fcc192_quinary", 2)])
+                                             ("fcc192_quinary", 2), ("fcc384_quaternary", 2),
+                                             ("fcc432_ternary", 2), ("fcc600_quinary", 2),
+                                             ("fcc768_ternary", 2)])
 def test_dense_shapes_match_sparse(gpu, name, ref_kernel):
     """explicit scoring of random candidates (bit-exact bonds, objectives and SRO against the
     sparse kernel, bonds against the oracle on a sample) and a short MC run whose pack equals

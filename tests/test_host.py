@@ -199,11 +199,26 @@ def test_kernel_shape_follows_the_applicability_rules():
     assert job.kernel_shape() == "wave"
     job.close()
     big = Job(parse_config(W.quinary_864(iterations=1000)))
-    assert big.kernel_shape().startswith("dense<28,")
+    assert big.kernel_shape() == "dense<27,1,2,7>"
     big.set_kernel(4)
     with pytest.raises(ValueError):
         big.kernel_shape()
     big.close()
+
+
+@pytest.mark.parametrize("sites,cell,bound", [(32, (2, 2, 2), 1), (96, (2, 3, 4), 4),
+                                              (192, (4, 4, 3), 6), (256, (4, 4, 4), 8),
+                                              (384, (4, 4, 6), 12), (432, (3, 4, 9), 14),
+                                              (500, (5, 5, 5), 16), (600, (5, 5, 6), 20),
+                                              (768, (4, 8, 6), 24), (864, (6, 6, 6), 27),
+                                              (1024, (8, 8, 4), 32)])
+def test_dense_tile_bound_by_cell_size(sites, cell, bound):
+    """the dense kernel runs the smallest tile bound that holds the cell's 32-site tiles"""
+    from sqsgenerator_amd.core import Job
+    comp = {"Fe": sites // 2, "Co": sites - sites // 2}
+    job = Job(parse_config(W.fcc(4.05, cell, comp, 3, iterations=1000, thread_config=1)))
+    assert job.kernel_shape().startswith(f"dense<{bound},")
+    job.close()
     sys16 = W.reference_test_16()
     del sys16["pair_weights"]
     syst = Job(parse_config(sys16))
