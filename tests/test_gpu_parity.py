@@ -613,3 +613,60 @@ def test_dense_shapes_match_sparse(gpu, name, ref_kernel):
         packs.append(_pack_signature(j.results()))
         j.close()
     assert packs[0] == packs[1]
+
+
+def _mid_cell(name, iterations):
+    d = WORKLOADS[name]() if name in WORKLOADS else _cell(name)
+    d["iterations"] = iterations
+    d["chunk_size"] = min(iterations, 100000)
+    return d
+
+
+MID_BOUNDS = ["fcc192_quinary", "fcc384_quaternary", "fcc432_ternary", "fcc600_quinary",
+              "fcc768_ternary", "fcc864_quinary"]  # tile bounds 6, 12, 14, 20, 24, 27
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_dense_mid_bounds_match_lane_kernel(gpu, use_graph):
+    """the persistent dense form over cells of six tile bounds (one launch group per bound,
+    the last launch of each job partial) returns the lane kernel's packs"""
+    print(f"[batch-dense-mid] graph={use_graph}", flush=True)
+    its = 20_000
+    alone = []
+    for name in MID_BOUNDS:
+        job = Job(parse_config(_mid_cell(name, its)))
+        job.set_kernel(2)
+        job.run()
+        alone.append(_pack_signature(job.results()))
+        job.close()
+    jobs = [Job(parse_config(_mid_cell(name, its))) for name in MID_BOUNDS]
+    for j in jobs:
+        j.set_kernel(3)
+        j.set_launch_size(8192)
+    shapes = [j.kernel_shape() for j in jobs]
+    assert [s.split(",")[0] for s in shapes] == ["dense<6", "dense<12", "dense<14", "dense<20",
+                                                "dense<24", "dense<27"]
+    rep = S.batch_run(jobs, use_graph=use_graph)
+    assert rep["candidates"] == its * len(jobs)
+    for j, ref in zip(jobs, alone):
+        assert j.timing()[2] == its
+        assert _pack_signature(j.results()) == ref
+        j.close()
+
+
+@pytest.mark.parametrize("iterations", [1, 241, 30_000])
+def test_paired_bound6_matches_lane_kernel(gpu, iterations):
+    """the paired form at tile bound 6 (192 sites, ring depth 7) returns the lane kernel's
+    pack"""
+    print(f"[paired-6] its={iterations}", flush=True)
+    packs = []
+    for kernel in (2, 4):
+        job = Job(parse_config(_mid_cell("fcc192_quinary", iterations)))
+        job.set_kernel(kernel)
+        if kernel == 4:
+            assert job.kernel_shape() == "paired<6,2,7>"
+        job.run()
+        assert job.timing()[2] == iterations
+        packs.append(_pack_signature(job.results()))
+        job.close()
+    assert packs[0] == packs[1]
