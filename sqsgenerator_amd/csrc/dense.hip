@@ -105,11 +105,11 @@ template <class F> hipError_t with_dense_shape(uint32_t tile_bound, F &&f) {
     case 6: return f(DenseShape<6, 2, 2, 3, 7>{});
     case 8: return f(DenseShape<8, 2, 2, 3, 9>{});
     case 12: return f(DenseShape<12, 2, 2, 3>{});
-    case 14: return f(DenseShape<14, 2, 2, 3>{});
+    case 14: return f(DenseShape<14, 2, 2, 5>{});
     case 16: return f(DenseShape<16, 2, 2, 2>{});
     case 20: return f(DenseShape<20, 1, 2, 7>{});
     case 24: return f(DenseShape<24, 1, 2, 6>{});
-    case 27: return f(DenseShape<27, 1, 2, 7>{});
+    case 27: return f(DenseShape<27, 1, 2, 9>{});
     case 32: return f(DenseShape<32, 1, 2, 4>{});
     default: return hipErrorInvalidValue;
   }

@@ -199,7 +199,7 @@ def test_kernel_shape_follows_the_applicability_rules():
     assert job.kernel_shape() == "wave"
     job.close()
     big = Job(parse_config(W.quinary_864(iterations=1000)))
-    assert big.kernel_shape() == "dense<27,1,2,7>"
+    assert big.kernel_shape() == "dense<27,1,2,9>"
     big.set_kernel(4)
     with pytest.raises(ValueError):
         big.kernel_shape()
