@@ -670,3 +670,58 @@ def test_paired_bound6_matches_lane_kernel(gpu, iterations):
         packs.append(_pack_signature(job.results()))
         job.close()
     assert packs[0] == packs[1]
+
+
+_SC_SPECIES = ["Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni"]
+
+
+def _sc_cell(dims, K, shells, seed):
+    """simple cubic supercell of any site count: K species as even as the count allows"""
+    n = int(np.prod(dims))
+    base, extra = divmod(n, K)
+    comp = {_SC_SPECIES[i]: base + (1 if i < extra else 0) for i in range(K)}
+    return dict(structure=dict(lattice=np.diag([3.0, 3.0, 3.0]).tolist(), coords=[[0, 0, 0]],
+                               species=["Al"], supercell=list(dims)),
+                composition=comp,
+                shell_weights={str(s): 1.0 / s for s in range(1, shells + 1)},
+                iterations=4096, chunk_size=4096, seed=seed, thread_config=1)
+
+
+@pytest.mark.parametrize("dims,K,shells,bound", [
+    ((3, 3, 3), 2, 3, 1), ((3, 3, 4), 2, 3, 2), ((3, 4, 6), 3, 4, 4), ((4, 5, 8), 4, 3, 6),
+    ((5, 5, 9), 5, 4, 8), ((5, 7, 10), 6, 3, 12), ((6, 7, 10), 7, 3, 14), ((6, 8, 10), 3, 4, 16),
+    ((7, 8, 11), 4, 3, 20), ((8, 9, 10), 2, 3, 24), ((7, 11, 11), 4, 3, 27),
+    ((8, 10, 11), 5, 3, 32), ((9, 10, 11), 3, 3, 32)])
+def test_dense_odd_cells_match_pair_list(gpu, dims, K, shells, bound):
+    """cells whose site count is no multiple of 32 (a partial last tile, and tiles past the
+    problem up to the bound) at every tile bound: explicit scoring by the dense kernel equals
+    the pair-list kernel bit for bit (bonds, objectives, SRO) and the oracle on a sample,
+    and a short Monte Carlo run returns the lane kernel's pack"""
+    n = int(np.prod(dims))
+    print(f"[dense-odd] {n} sites K={K} S={shells}", flush=True)
+    cfg = parse_config(_sc_cell(dims, K, shells, n))
+    assert not isinstance(cfg, S.ParseError), cfg
+    job = Job(cfg)
+    assert job.kernel_shape().startswith(f"dense<{bound},")
+    ref = oracle_inputs(cfg, job)
+    rng = np.random.default_rng(n)
+    B = 128 if n < 500 else 32
+    cands = random_candidates(ref["packed"], ref["bounds"], B, rng)
+    b_w, o_w, s_w = job.score(cands, 0, kernel=1)
+    b_d, o_d, s_d = job.score(cands, 0, kernel=3)
+    assert np.array_equal(b_w, b_d)
+    assert np.array_equal(o_w.view(np.uint64), o_d.view(np.uint64))
+    assert np.array_equal(s_w, s_d)
+    for b in rng.choice(B, 4, replace=False):
+        rb = O.count_bonds(ref["pairs"], cands[b], ref["S"], ref["K"])
+        assert np.array_equal(b_d[b].astype(np.uint64), rb)
+    job.close()
+    packs = []
+    for kernel in (2, 3):
+        j = Job(parse_config(_sc_cell(dims, K, shells, n)))
+        j.set_kernel(kernel)
+        j.run()
+        assert j.timing()[2] == 4096
+        packs.append(_pack_signature(j.results()))
+        j.close()
+    assert packs[0] == packs[1]
