@@ -672,7 +672,8 @@ def test_paired_bound6_matches_lane_kernel(gpu, iterations):
     assert packs[0] == packs[1]
 
 
-_SC_SPECIES = ["Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni"]
+_SC_SPECIES = ["Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Zr", "Nb", "Mo", "Ru",
+               "Rh", "Pd", "Ag"]
 
 
 def _sc_cell(dims, K, shells, seed):
@@ -691,12 +692,14 @@ def _sc_cell(dims, K, shells, seed):
     ((3, 3, 3), 2, 3, 1), ((3, 3, 4), 2, 3, 2), ((3, 4, 6), 3, 4, 4), ((4, 5, 8), 4, 3, 6),
     ((5, 5, 9), 5, 4, 8), ((5, 7, 10), 6, 3, 12), ((6, 7, 10), 7, 3, 14), ((6, 8, 10), 3, 4, 16),
     ((7, 8, 11), 4, 3, 20), ((8, 9, 10), 2, 3, 24), ((7, 11, 11), 4, 3, 27),
-    ((8, 10, 11), 5, 3, 32), ((9, 10, 11), 3, 3, 32)])
+    ((8, 10, 11), 5, 3, 32), ((9, 10, 11), 3, 3, 32), ((5, 7, 10), 11, 3, 12),
+    ((7, 8, 11), 16, 2, 20), ((8, 10, 11), 13, 2, 32)])
 def test_dense_odd_cells_match_pair_list(gpu, dims, K, shells, bound):
     """cells whose site count is no multiple of 32 (a partial last tile, and tiles past the
     problem up to the bound) at every tile bound: explicit scoring by the dense kernel equals
     the pair-list kernel bit for bit (bonds, objectives, SRO) and the oracle on a sample,
-    and a short Monte Carlo run returns the lane kernel's pack"""
+    and a short Monte Carlo run returns the lane kernel's pack (the wave kernel's beyond the
+    lane kernel's 8 species)"""
     n = int(np.prod(dims))
     print(f"[dense-odd] {n} sites K={K} S={shells}", flush=True)
     cfg = parse_config(_sc_cell(dims, K, shells, n))
@@ -717,7 +720,7 @@ def test_dense_odd_cells_match_pair_list(gpu, dims, K, shells, bound):
         assert np.array_equal(b_d[b].astype(np.uint64), rb)
     job.close()
     packs = []
-    for kernel in (2, 3):
+    for kernel in (2 if K <= 8 else 1, 3):
         j = Job(parse_config(_sc_cell(dims, K, shells, n)))
         j.set_kernel(kernel)
         j.run()
