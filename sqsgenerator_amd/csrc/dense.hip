@@ -910,14 +910,24 @@ template <class Sh> const void *ws_fn(bool persistent) {
                     : reinterpret_cast<const void *>(&run_dense_ws_kernel<Sh::TT, Sh::G, ws_ring<Sh>()>);
 }
 
-template <class Sh> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s) {
-  const void *fn = reinterpret_cast<const void *>(&score_dense_kernel<Sh::TT, Sh::G, Sh::DS>);
+template <int TT, int G, int DS> hipError_t score_dense_g(const ScoreParams &p, hipStream_t s) {
+  const void *fn = reinterpret_cast<const void *>(&score_dense_kernel<TT, G, DS>);
   const hipError_t e = dense_lds(fn, p.wave_lds);
   if (e != hipSuccess) return e;
   const uint32_t blocks = static_cast<uint32_t>((p.B + p.dense_gen - 1) / p.dense_gen);
-  hipLaunchKernelGGL((score_dense_kernel<Sh::TT, Sh::G, Sh::DS>), dim3(blocks), dim3(64),
-                     p.wave_lds, s, p);
+  hipLaunchKernelGGL((score_dense_kernel<TT, G, DS>), dim3(blocks), dim3(64), p.wave_lds, s,
+                     p);
   return hipGetLastError();
+}
+
+// the shape's tile groups, or one group per wavefront for batches too small to fill the
+// device (the host then passes a one-group pass width)
+template <class Sh> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s) {
+  if constexpr (Sh::G > 1) {
+    if (p.dense_groups == 1) return score_dense_g<Sh::TT, 1, Sh::DS>(p, s);
+  }
+  if (p.dense_groups != static_cast<uint32_t>(Sh::G)) return hipErrorInvalidValue;
+  return score_dense_g<Sh::TT, Sh::G, Sh::DS>(p, s);
 }
 
 }  // namespace

@@ -125,6 +125,7 @@ struct ScoreParams {
   double *sro;                 // B x S x K x K or null
   uint32_t wave_lds, off_row, off_hist, off_terms;
   uint32_t dense_cw, dense_gen, dense_stride;
+  uint32_t dense_groups;       // tile groups per wavefront of the dense kernel: 1 or the shape's
 };
 
 }  // namespace sqsg

@@ -845,11 +845,19 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     const RunParams rp = make_run_params(job, SQSG_KERNEL_DENSE_MFMA, 1, &blocks);
     sp.prob = P.dev();
     sp.dense_cw = rp.dense_cw;
+    sp.dense_groups = dense_groups(dense_tile_bound_of(job));
     // candidates per wavefront: whole passes, fewer of them when the batch would otherwise
-    // leave most SIMDs idle (aim: two wavefronts per SIMD)
+    // leave most SIMDs idle (aim: two wavefronts per SIMD); a batch that does not fill them
+    // with one pass per wavefront runs one tile group per wavefront (twice the wavefronts,
+    // each with half the MFMA chain)
     uint32_t gen = rp.gen_group;
     const uint64_t want = uint64_t(8) * static_cast<uint64_t>(device_cu_count());
     while (gen > rp.dense_cw && (uint64_t(B) + gen - 1) / gen < want) gen -= rp.dense_cw;
+    if (gen == rp.dense_cw && sp.dense_groups > 1 && (uint64_t(B) + gen - 1) / gen < want) {
+      sp.dense_cw = rp.dense_cw / sp.dense_groups;
+      sp.dense_groups = 1;
+      gen = sp.dense_cw;
+    }
     sp.dense_gen = gen;
     sp.dense_stride = rp.dense_stride;
     sp.off_row = rp.off_rows[0];
