@@ -728,3 +728,29 @@ def test_dense_odd_cells_match_pair_list(gpu, dims, K, shells, bound):
         packs.append(_pack_signature(j.results()))
         j.close()
     assert packs[0] == packs[1]
+
+
+@pytest.mark.parametrize("name,B", [("fcc108_ternary", 49152), ("fcc256_quinary", 32768)])
+def test_score_dense_large_batch_tile_groups(gpu, name, B):
+    """explicit scoring of a batch that fills the device at one pass per wavefront runs the
+    shape's tile groups per wavefront; a small batch runs one group: both equal the pair-list
+    kernel bit for bit, and each other on the candidates they share"""
+    print(f"[score-large-batch] {name} B={B}", flush=True)
+    cfg = parse_config(WORKLOADS[name]())
+    job = Job(cfg)
+    ref = oracle_inputs(cfg, job)
+    rng = np.random.default_rng(B)
+    cands = random_candidates(ref["packed"], ref["bounds"], B, rng)
+    b_w, o_w, s_w = job.score(cands, 0, kernel=1)
+    b_d, o_d, s_d = job.score(cands, 0, kernel=3)
+    assert np.array_equal(b_w, b_d)
+    assert np.array_equal(o_w.view(np.uint64), o_d.view(np.uint64))
+    assert np.array_equal(s_w, s_d)
+    b_s, o_s, s_s = job.score(cands[:4096], 0, kernel=3)
+    assert np.array_equal(b_s, b_d[:4096])
+    assert np.array_equal(o_s.view(np.uint64), o_d[:4096].view(np.uint64))
+    assert np.array_equal(s_s, s_d[:4096])
+    for b in rng.choice(B, 8, replace=False):
+        rb = O.count_bonds(ref["pairs"], cands[b], ref["S"], ref["K"])
+        assert np.array_equal(b_d[b].astype(np.uint64), rb)
+    job.close()
