@@ -79,7 +79,7 @@ class RunConfig(C.Structure):
                 ("sublattices", C.POINTER(Sublattice)), ("iterations", C.c_uint64),
                 ("chunk_size", C.c_uint64), ("keep", C.c_uint64),
                 ("max_results_per_objective", C.c_int64), ("num_threads", C.c_uint32),
-                ("shared_rng", C.c_int), ("collect", C.c_int)]
+                ("shared_rng", C.c_int), ("collect", C.c_int), ("counter_stream", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -315,13 +315,17 @@ class Problem:
 
 
 def run(problems, iterations, chunk_size=None, systematic=False, split=False, keep=1,
-        max_results_per_objective=None, threads=1, shared_rng=True, collect=True):
-    """sqs.h:299-467 worker loop; returns (stats, [(objective, species, sub_objectives)])"""
+        max_results_per_objective=None, threads=1, shared_rng=True, collect=True,
+        counter_stream=False):
+    """sqs.h:299-467 worker loop; returns (stats, [(objective, species, sub_objectives)]).
+    counter_stream: Monte Carlo candidate t is the sorted configuration shuffled with draws
+    [t * D, (t + 1) * D) of each sublattice's seed stream (the device path's candidate
+    definition) instead of the reference's cumulative shuffles"""
     arr = (Sublattice * len(problems))(*[p.struct() for p in problems])
     cfg = RunConfig(int(systematic), int(split), len(problems), arr, iterations,
                     chunk_size or min(iterations, 100000), keep,
                     -1 if max_results_per_objective is None else max_results_per_objective,
-                    threads, int(shared_rng), int(collect))
+                    threads, int(shared_rng), int(collect), int(counter_stream))
     st = RunStats()
     if lib().or_run(C.byref(cfg), C.byref(st)):
         raise ValueError("invalid run configuration")

@@ -639,8 +639,21 @@ static void *worker(void *arg) {
       local_rng[q] = cfg->sublattices[q].seed + rstart * 0x9e3779b97f4a7c15ull;
     }
     if (cfg->systematic) unrank_window(species, &cfg->sublattices[0], (u128)rstart + 1);
+    int stalled = 0; /* interact: a max_results skip left the configuration unchanged */
     for (uint64_t it = rstart; it < rend; ++it) {
       double objective_value;
+      if (cfg->counter_stream && !cfg->systematic && !stalled) {
+        for (uint32_t q = 0; q < nsub; ++q) {
+          const or_sublattice *sl = &cfg->sublattices[q];
+          memcpy(species + off[q], sl->species_packed, sl->num_sites);
+          if (!cfg->split && it == rstart) continue; /* sqs.h:374,400-402 */
+          uint64_t draws = 0;
+          for (uint32_t b = 0; b < sl->num_bounds; ++b)
+            draws += sl->bounds[2 * b + 1] - sl->bounds[2 * b] - 1;
+          uint64_t st = sl->seed + it * draws * RAPID_SECRET0;
+          shuffle_window(species + off[q], sl, 0, &st);
+        }
+      }
       if (!cfg->split) {
         const or_sublattice *sl = &cfg->sublattices[0];
         or_count_bonds(sl->pairs, sl->num_pairs, species, sl->num_shells, sl->num_species, bonds[0]);
@@ -650,7 +663,9 @@ static void *worker(void *arg) {
       } else {
         for (uint32_t q = 0; q < nsub; ++q) {
           const or_sublattice *sl = &cfg->sublattices[q];
-          shuffle_window(species + off[q], sl, cfg->shared_rng ? &ctx->rng[q] : 0, &local_rng[q]);
+          if (!cfg->counter_stream)
+            shuffle_window(species + off[q], sl, cfg->shared_rng ? &ctx->rng[q] : 0,
+                           &local_rng[q]);
           or_count_bonds(sl->pairs, sl->num_pairs, species + off[q], sl->num_shells,
                          sl->num_species, bonds[q]);
           subobj[q] = or_compute_objective(bonds[q], sl->prefactors, sl->pair_weights, sl->target,
@@ -673,14 +688,17 @@ static void *worker(void *arg) {
           if (nb < bits2d(atomic_load(&ctx->search_bits))) atomic_store(&ctx->search_bits, d2bits(nb));
         }
         pthread_mutex_unlock(&g_coll.lock);
-        if (skip) continue; /* sqs.h:423-426: skips the shuffle below as well */
+        if (skip) { /* sqs.h:423-426: skips the shuffle below as well */
+          if (!cfg->split) stalled = 1;
+          continue;
+        }
       }
       if (!cfg->split) {
         const or_sublattice *sl = &cfg->sublattices[0];
         if (cfg->systematic) {
           uint32_t lo = sl->bounds[0], hi = sl->bounds[1];
           or_next_permutation(species + lo, hi - lo);
-        } else
+        } else if (!cfg->counter_stream)
           shuffle_window(species, sl, cfg->shared_rng ? &ctx->rng[0] : 0, &local_rng[0]);
       }
     }

@@ -33,9 +33,15 @@
  *   or_count_bonds         brute-force dense contraction X^T U X; per-shell totals = pairs
  *   or_compute_objective   closed forms, zero weights, unit prefactors, summation order
  *   or_run                 systematic: all 12870 results with zero pair weights; keep
- *                          levels hold every tie; max_results_per_objective allows max+1.
- *                          Membership of non-optimal MC levels is PARITY UNPINNED (it
- *                          depends on the unpinned trajectory).
+ *                          levels hold every tie; max_results_per_objective allows max+1;
+ *                          the whole collection (every level, never evicted, the stall of
+ *                          a block after a max_results skip) equals a plain-Python
+ *                          statement of the one-thread worker loop, in systematic and in
+ *                          counter_stream mode, over BS::thread_pool's block split.
+ *                          counter_stream: with keep above the candidate count the pack is
+ *                          the candidate set built from the golden-pinned shuffle and draw
+ *                          stream.  The reference's own MC trajectory (cumulative shuffles
+ *                          of a shared RNG state) stays PARITY UNPINNED.
  */
 #ifndef SQSGEN_ORACLE_H
 #define SQSGEN_ORACLE_H
@@ -114,6 +120,11 @@ typedef struct {
   uint32_t num_threads;
   int shared_rng;        /* 1: one RNG state shared by all threads (reference behaviour) */
   int collect;           /* 1: keep the result collection (tests); 0: count only (timing) */
+  /* 1: Monte Carlo candidate `it` is the sorted configuration shuffled with the draws
+   * [it * D, (it + 1) * D) of the seed's stream (D = draws per candidate), the first of an
+   * interact block left unshuffled -- the counter-based candidate definition of the device
+   * path; 0: the reference's cumulative shuffles of one running configuration */
+  int counter_stream;
 } or_run_config;
 
 typedef struct {

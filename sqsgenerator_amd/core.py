@@ -712,17 +712,21 @@ class Job:
             L.check(L.lib().sqsg_run(self.handle, L.PROGRESS_FN(), None))
             return
         job = self
+        raised = []
 
         def trampoline(stats_p, _user):
             ctx = SqsCallbackContext(_stats_dict(stats_p.contents))
             try:
                 callback(ctx)
-            except Exception:
+            except BaseException as e:  # stop the run, re-raised once sqsg_run returns
+                raised.append(e)
                 return 1
             return 1 if ctx._stop else 0
 
         fn = L.PROGRESS_FN(trampoline)
         L.check(L.lib().sqsg_run(job.handle, fn, None))
+        if raised:
+            raise raised[0]
 
     def cancel(self) -> None:
         L.lib().sqsg_cancel(self.handle)
@@ -952,6 +956,13 @@ class SqsResultPack:
                                          L.ptr(sp, L.u8p) if species else None,
                                          L.ptr(rk, L.u64p) if ranks else None))
         return dict(objective=obj, candidate=cand, species=sp, rank=rk)
+
+    def species_packed(self, r: int) -> np.ndarray:
+        """internal form of result r: packed species indices in sorted-site order,
+        concatenated over the problems (the configuration the kernels scored)"""
+        out = np.empty(sum(p["num_sites"] for p in self._problems), dtype=np.uint8)
+        L.check(L.lib().sqsg_result_species_packed(self._job.handle, r, L.ptr(out, L.u8p)))
+        return out
 
     def sro_array(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
         count = self._n - first if count is None else count

@@ -75,6 +75,8 @@ struct Slot {
   HostCounters *hc = nullptr;
   uint64_t next = 0;          // first candidate not issued yet
   uint64_t slice = 0;         // candidates per step
+  uint32_t ramp = 0;          // steps issued so far
+  bool ramp_up = false;       // early steps are shorter (records not bounded by the filter)
   uint64_t chunk = 0;         // candidates a wavefront claims at once (one segment)
   std::vector<Range> issued[2];  // ranges of the step of each parity
   std::deque<Range> retry;    // ranges to issue again (their record sink overflowed)
@@ -191,10 +193,12 @@ class BatchEngine {
     s->count = s->buf->count;
     s->scored = s->buf->scored;
     s->hc = s->buf->hc;
-    const uint32_t cap = static_cast<uint32_t>(std::min<uint64_t>(s->slice, 1u << 14));
+    const uint32_t cap =
+        first_sink_records(job, uint64_t(device_cu_count()) * 8, s->slice);
+    s->ramp_up = !records_bounded(job);
     for (int q = 0; q < 2; ++q) s->sink[q].ensure(cap, job->stride, s->p.nprob);
     *s->hc = HostCounters{};
-    s->hc->thr = objective_key(std::numeric_limits<double>::infinity());
+    s->hc->thr = search_key(job);
     SQSG_HIP(hipMemcpyAsync(s->buf->thr.as<void>(), &s->hc->thr, sizeof(s->hc->thr),
                             hipMemcpyHostToDevice, stream_));
     s->p.thr_key = s->buf->thr.as<unsigned long long>();
@@ -281,8 +285,6 @@ class BatchEngine {
                               hipMemcpyDeviceToHost, copy_));
       SQSG_HIP(hipMemcpyAsync(&s->hc->scored[q], s->scored[q].as<void>(),
                               sizeof(unsigned long long), hipMemcpyDeviceToHost, copy_));
-      SQSG_HIP(hipMemcpyAsync(&s->hc->thr, s->buf->thr.as<void>(), sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, copy_));
     }
     SQSG_HIP(hipStreamSynchronize(copy_));
     for (auto &s : slots_) {
@@ -303,10 +305,10 @@ class BatchEngine {
             throw DeviceError("persistent launch scored an unexpected candidate count");
           job->scored += expect;
           candidates += expect;
-          s->sink[q].drain(job, count, s->p.nprob, copy_);
-          // keep the device threshold at the tighter of its own bound and the exact host
-          // one; ordered before the next step (a step running now only lowers it further)
-          const unsigned long long hk = objective_key(job->vkeep);
+          s->sink[q].drain(job, count, s->p.nprob, s->issued[q], copy_);
+          // the host's threshold after the candidates merged so far, ordered before the
+          // next step (the step running now keeps the older, looser one)
+          const unsigned long long hk = search_key(job);
           if (hk < s->hc->thr) {
             s->hc->thr = hk;
             SQSG_HIP(hipMemcpyAsync(s->buf->thr.as<void>(), &s->hc->thr, sizeof(s->hc->thr),
@@ -376,7 +378,15 @@ class BatchEngine {
         s->retry.pop_front();
       }
       if (s->next < s->job->iterations && !s->job->cancel.load()) {
-        const uint64_t end = std::min(s->job->iterations, s->next + s->slice);
+        // without the wavefront filter the device records every candidate at or below the
+        // threshold as it stood before the step: such steps grow from 4096 candidates by 4x
+        // up to the slice
+        const uint64_t size =
+            s->ramp_up ? std::min(s->slice, std::max(s->chunk, uint64_t(4096)
+                                                                   << std::min(2 * s->ramp, 40u)))
+                       : s->slice;
+        ++s->ramp;
+        const uint64_t end = std::min(s->job->iterations, s->next + size);
         s->issued[q].push_back(Range{s->next, end});
         s->next = end;
       }

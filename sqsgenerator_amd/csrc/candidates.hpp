@@ -102,14 +102,29 @@ __device__ void next_perm_candidate(Strided<SH> v, uint32_t lo, uint32_t w) {
 }
 
 
-// Offers the objectives of the lanes in `pass` (a ballot) to the filter and lowers the
-// job's threshold when the wave holds L levels below it.  `g` is the threshold read
-// before the candidates were tested.
+// First candidate of a block of the reference's thread-pool split (sqs.h:461-466; the
+// pool's blocks are near-equal with the first `remainder` ones one larger), which an
+// interact Monte Carlo run scores unshuffled (sqs.h:374,400-402,439-440).
+template <class RP>
+__device__ __forceinline__ bool block_start(const RP &p, uint64_t cand) {
+  return cand < p.blk_split ? cand % (p.blk_size + 1) == 0
+                            : (cand - p.blk_split) % p.blk_size == 0;
+}
+
+// Record bound of the wavefront filter as it stands (before the current candidates are
+// offered, so it holds only candidates that precede them): the order key of its L-th level,
+// or no bound.
+template <class RP>
+__device__ __forceinline__ unsigned long long filter_bound(const LevelFilter &lf,
+                                                           const RP &p) {
+  return p.wave_filter && lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
+}
+
+// Offers the objectives of the lanes in `pass` (a ballot) to the filter.
 template <class RP>
 __device__ __forceinline__ void offer_levels(LevelFilter &lf, const RP &p,
-                                             unsigned long long pass, double tot,
-                                             unsigned long long g, int lane) {
-  bool changed = false;
+                                             unsigned long long pass, double tot) {
+  if (!p.wave_filter) return;
   while (pass) {
     const int l = __ffsll(pass) - 1;
     pass &= pass - 1;
@@ -123,11 +138,6 @@ __device__ __forceinline__ void offer_levels(LevelFilter &lf, const RP &p,
       --pos;
     }
     lf.lvl[pos] = o;
-    changed = true;
-  }
-  if (changed && lf.n == p.L && lane == 0) {
-    const unsigned long long k = okey(lf.lvl[p.L - 1]);
-    if (k < g) atomicMin(p.thr_key, k);
   }
 }
 

@@ -397,7 +397,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
         }
       } else {
         load_dense_row<TT>(row, pr);
-        if (active && !(p.chunk_quirk && cand % p.chunk_size == 0))
+        if (active && !(p.chunk_quirk && block_start(p, cand)))
           shuffle_candidate(Strided<0>{row}, pr, cand);
       }
     }
@@ -407,27 +407,20 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
     for (uint32_t q = 0; q < npass; ++q)
       if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
     wave_sync();
-    // early exit bound: the job threshold and the wavefront's own L-th level as they stand
-    // now (both only decrease, so they stay valid bounds for the record test below)
-    unsigned long long prune_key = ~0ull;
-    if (p.prune) {
-      const unsigned long long g0 =
-          __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned long long l0 = lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
-      prune_key = g0 < l0 ? g0 : l0;
-    }
+    // the job threshold and the wavefront filter's bound before this round: both hold for
+    // every candidate of the round, so they serve the early exit and the record test
+    const unsigned long long g =
+        __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long lk = filter_bound(lf, p);
+    const unsigned long long prune_key = p.prune ? (g < lk ? g : lk) : ~0ull;
     const double tot =
         dense_score<TT, G, D, kDenseTermTables<TT>>(pr, rows, stride, cnt, CW, npass, live,
                                                     lane, nullptr, nullptr, 0u, am, prune_key);
     scored += static_cast<unsigned long long>(__popcll(am));
-    const unsigned long long g =
-        __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // the wave's own L-th distinct level bounds the final levels as well: a candidate above
-    // it has L better distinct objectives and cannot be kept, so it is not recorded
+    // a candidate above the threshold or above the filter's L-th level cannot be inserted
     const bool below = active && okey(tot) <= g;
-    offer_levels(lf, p, __ballot(below), tot, g, lane);
-    const unsigned long long lk = lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
     const bool pass = below && okey(tot) <= lk;
+    offer_levels(lf, p, __ballot(below), tot);
     if (pass) {
       const uint32_t slot = atomicAdd(p.rec_count, 1u);
       if (slot < p.rec_cap) {
@@ -718,10 +711,11 @@ __device__ __forceinline__ void ws_record(const RP &p, const uint8_t *rows, uint
   if (lane == 0) atomicAdd(p.scored, static_cast<unsigned long long>(__popcll(am)));
   const unsigned long long g =
       __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long lk = filter_bound(lf, p);
   const bool below = active && okey(tot) <= g;
-  offer_levels(lf, p, __ballot(below), tot, g, lane);
-  const unsigned long long lk = lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
-  if (below && okey(tot) <= lk) {
+  const bool pass = below && okey(tot) <= lk;
+  offer_levels(lf, p, __ballot(below), tot);
+  if (pass) {
     const uint32_t slot = atomicAdd(p.rec_count, 1u);
     if (slot < p.rec_cap) {
       const uint8_t *row = rows + static_cast<uint32_t>(lane) * p.dense_stride;
@@ -742,7 +736,7 @@ __device__ __forceinline__ void ws_start(const RP &p, const WsBatch &bt, uint32_
   const auto &pr = p.prob[0];
   load_dense_row<TT>(rows + static_cast<uint32_t>(lane) * p.dense_stride, pr);
   const uint64_t cand = bt.cbeg + pair * p.gen_group + static_cast<uint32_t>(lane);
-  fy_init(f, pr, cand, cand < bt.cend && !(p.chunk_quirk && cand % p.chunk_size == 0));
+  fy_init(f, pr, cand, cand < bt.cend && !(p.chunk_quirk && block_start(p, cand)));
 }
 
 template <int TT, int G, int D, class Source>

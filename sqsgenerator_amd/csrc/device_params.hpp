@@ -7,7 +7,7 @@ namespace sqsg {
 
 constexpr int kMaxProblems = 4;   // split-mode sublattices handled in one candidate
 constexpr int kMaxBounds = 8;     // shuffle windows per problem (interact sublattices)
-constexpr int kMaxLevels = 16;    // distinct objective levels tracked on device (keep + 1)
+constexpr int kMaxLevels = 16;    // levels of the wavefront filter (keep + 1 at most)
 constexpr int kMaxLocalSpecies = 16;
 constexpr uint32_t kPairIndexBits = 12;  // packed pair: i | j << 12 | shell << 24
 constexpr uint32_t kMaxSites = 1u << kPairIndexBits;
@@ -59,8 +59,10 @@ struct DevProblem {
 };
 
 // Wave-local state of the tie-keeping filter: the L smallest distinct objectives a
-// wavefront has seen for its current job; its L-th bounds the job's global threshold.
-// The dense kernels keep it in LDS, one copy per wavefront.
+// wavefront has seen for its current job.  A candidate above the L-th of them has L
+// smaller distinct objectives before it in the job's candidate order, so the reference
+// would not insert it (sqs.h:421 against results.h:78-82).  The dense kernels keep it in
+// LDS, one copy per wavefront.
 struct LevelFilter {
   double lvl[kMaxLevels];
   uint32_t n;
@@ -70,11 +72,18 @@ struct RunParams {
   DevProblem prob[kMaxProblems];
   uint32_t nprob;
   uint32_t systematic;
-  uint32_t chunk_quirk;        // interact MC: candidate t with t % chunk == 0 is unshuffled
+  uint32_t chunk_quirk;        // interact MC: the first candidate of every block is unshuffled
   uint32_t rounds;             // systematic: consecutive ranks walked per lane
-  uint64_t chunk_size;
+  // blocks of the reference's thread pool split (near-equal, the first ones one larger):
+  // candidates below blk_split form blocks of blk_size + 1, the rest blocks of blk_size
+  uint64_t blk_size, blk_split;
   uint64_t cand_begin, cand_end;
-  uint32_t L;                  // distinct objective levels kept (keep + 1)
+  uint32_t L;                  // distinct objective levels of the wavefront filter (keep + 1)
+  // the wavefront filter bounds the records: set where a wavefront sees its candidates in
+  // ascending order and the reference's insertion rule depends on objectives alone (Monte
+  // Carlo runs other than interact runs with max_results_per_objective; keep + 1 <=
+  // kMaxLevels)
+  uint32_t wave_filter;
   uint32_t gen_group;          // candidates generated per wave and round (<= 64)
   uint32_t dense_cw;           // dense: candidates per MFMA pass (G groups of 32 / K)
   uint32_t dense_stride;       // dense: bytes of one LDS candidate row
@@ -85,7 +94,8 @@ struct RunParams {
   uint32_t off_levels;         // dense: the wavefront's tie filter levels (LevelFilter)
   uint32_t ws_pair;            // dense paired form: LDS bytes of one pair's region
   uint32_t prune;              // dense: stop a pass once no candidate can reach the kept levels
-  // tie-keeping threshold and record sink
+  // the host's search threshold (order key of the keep-th distinct inserted objective as of
+  // the candidates merged before this launch; kernels only read it) and the record sink
   unsigned long long *thr_key;
   unsigned int *rec_count;
   uint64_t *rec_cand;

@@ -385,8 +385,6 @@ sqsg_job *create_job(const sqsg_config &c) {
     throw std::invalid_argument("In \"systematic\" iteration mode only one sublattice is allowed");
   if (c.keep == 0)
     throw std::invalid_argument("The number of best kept structures must be greater than 0");
-  if (c.keep + 1 > static_cast<uint64_t>(kMaxLevels))
-    throw std::out_of_range("keep above 15 distinct objectives is not supported on device");
   if (c.iterations == 0) throw std::invalid_argument("iterations must be positive");
   if (c.chunk_size == 0 || c.chunk_size > c.iterations)
     throw std::out_of_range("\"chunk_size\" must be in [1, iterations]");
@@ -396,6 +394,12 @@ sqsg_job *create_job(const sqsg_config &c) {
   job->num_sites = N;
   job->iterations = c.iterations;
   job->chunk_size = c.chunk_size;
+  {  // pool.detach_blocks(0, iterations, worker, iterations / chunk_size) (sqs.h:461-466):
+     // BS::thread_pool's near-equal blocks, the first `iterations % blocks` one larger
+    const uint64_t nb = std::max<uint64_t>(1, c.iterations / c.chunk_size);
+    job->blk_size = c.iterations / nb;
+    job->blk_split = (c.iterations % nb) * (job->blk_size + 1);
+  }
   job->keep = c.keep;
   job->max_results = c.max_results_per_objective;
   Lattice lat;
@@ -559,6 +563,23 @@ int resolve_kernel(const sqsg_job *job, int requested) {
   }
 }
 
+// The wavefront filter bounds records where the reference's insertion of a candidate depends
+// on the objectives before it alone: not in systematic runs (a wavefront's lanes walk
+// interleaved rank ranges), not where a skip for max_results_per_objective stalls an interact
+// block (the stalled candidates' objectives are not the reference's).
+bool records_bounded(const sqsg_job *job) {
+  return job->iteration_mode != SQSG_ITERATION_SYSTEMATIC &&
+         job->keep + 1 <= static_cast<uint64_t>(kMaxLevels) &&
+         (job->max_results < 0 || job->sublattice_mode == SQSG_SUBLATTICE_SPLIT);
+}
+
+uint32_t first_sink_records(const sqsg_job *job, uint64_t waves, uint64_t launch) {
+  // with the filter, a wavefront records about L ln(n) + ties of its n candidates while the
+  // threshold is still loose
+  const uint64_t want = std::max<uint64_t>(uint64_t(1) << 14, waves * (job->keep + 1) * 16);
+  return static_cast<uint32_t>(std::min<uint64_t>({launch, want, uint64_t(1) << 22}));
+}
+
 RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *blocks) {
   RunParams p{};
   p.nprob = static_cast<uint32_t>(job->problems.size());
@@ -572,8 +593,10 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   }
   p.systematic = job->iteration_mode == SQSG_ITERATION_SYSTEMATIC;
   p.chunk_quirk = (!p.systematic && job->sublattice_mode == SQSG_SUBLATTICE_INTERACT) ? 1u : 0u;
-  p.chunk_size = job->chunk_size;
-  p.L = static_cast<uint32_t>(job->keep + 1);
+  p.blk_size = job->blk_size;
+  p.blk_split = job->blk_split;
+  p.L = static_cast<uint32_t>(std::min<uint64_t>(job->keep + 1, kMaxLevels));
+  p.wave_filter = records_bounded(job) ? 1u : 0u;
   p.prune = job->prune ? 1u : 0u;
   auto align = [](uint32_t x) { return (x + 15u) & ~15u; };
   uint32_t off = 0;
@@ -689,27 +712,38 @@ void RecordSink::bind(RunParams &p) const {
   p.rec_species = species.as<uint8_t>();
 }
 
-void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob, hipStream_t s) {
-  if (!count) return;
+void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob,
+                       const std::vector<std::pair<uint64_t, uint64_t>> &ranges,
+                       hipStream_t s) {
   const uint32_t stride = job->stride;
   std::vector<uint64_t> c(count);
   std::vector<double> o(count), sb(size_t(count) * nprob);
   std::vector<uint8_t> sp(size_t(count) * stride);
-  SQSG_HIP(hipMemcpyAsync(c.data(), cand.as<void>(), count * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-  SQSG_HIP(hipMemcpyAsync(o.data(), obj.as<void>(), count * sizeof(double),
-                          hipMemcpyDeviceToHost, s));
-  SQSG_HIP(hipMemcpyAsync(sb.data(), sub.as<void>(), sb.size() * sizeof(double),
-                          hipMemcpyDeviceToHost, s));
-  SQSG_HIP(hipMemcpyAsync(sp.data(), species.as<void>(), sp.size(), hipMemcpyDeviceToHost, s));
-  SQSG_HIP(hipStreamSynchronize(s));
-  merge_records(job, c.data(), o.data(), sb.data(), sp.data(), count);
-  for (uint32_t k = 0; k < count; ++k)
-    if (o[k] < job->stats.best_objective ||
-        (o[k] == job->stats.best_objective && c[k] < job->stats.best_rank)) {
-      job->stats.best_objective = o[k];
-      job->stats.best_rank = c[k];
+  if (count) {
+    SQSG_HIP(hipMemcpyAsync(c.data(), cand.as<void>(), count * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipMemcpyAsync(o.data(), obj.as<void>(), count * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipMemcpyAsync(sb.data(), sub.as<void>(), sb.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipMemcpyAsync(sp.data(), species.as<void>(), sp.size(), hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipStreamSynchronize(s));
+  }
+  // one record set per candidate range (a step can hold a re-issued range and the next one)
+  for (const auto &[begin, end] : ranges) {
+    HostRecords r;
+    r.begin = begin;
+    r.end = end;
+    for (uint32_t k = 0; k < count; ++k) {
+      if (c[k] < begin || c[k] >= end) continue;
+      r.cand.push_back(c[k]);
+      r.obj.push_back(o[k]);
+      r.sub.insert(r.sub.end(), sb.begin() + size_t(k) * nprob, sb.begin() + size_t(k + 1) * nprob);
+      r.species.insert(r.species.end(), sp.begin() + size_t(k) * stride,
+                       sp.begin() + size_t(k + 1) * stride);
     }
+    merge_records(job, std::move(r));
+  }
 }
 
 uint64_t default_launch_size(const sqsg_job *job) {
@@ -740,19 +774,14 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   uint32_t blocks = 0;
   RunParams p = make_run_params(job, variant, W, &blocks);
   const uint64_t launch = job->launch_size ? job->launch_size : default_launch_size(job);
-  if (p.systematic) {
-    const uint64_t lanes = uint64_t(blocks) * W * p.gen_group;
-    p.rounds = static_cast<uint32_t>(std::clamp<uint64_t>(launch / (lanes * 2), 1, 1024));
-  } else {
-    p.rounds = 1;
-  }
   const uint32_t stride = job->stride;
   p.rec_stride = stride;
   RecordSink sink;
-  sink.ensure(static_cast<uint32_t>(std::min<uint64_t>(launch, 1u << 16)), stride, p.nprob);
+  sink.ensure(first_sink_records(job, uint64_t(blocks) * W, launch), stride, p.nprob);
+  const bool ramp_up = !records_bounded(job);
   DevBuf d_thr(sizeof(unsigned long long)), d_count(sizeof(unsigned int)),
       d_scored(sizeof(unsigned long long)), d_next(sizeof(unsigned int));
-  unsigned long long thr = objective_key(std::numeric_limits<double>::infinity());
+  unsigned long long thr = search_key(job);
   SQSG_HIP(hipMemcpy(d_thr.as<void>(), &thr, sizeof(thr), hipMemcpyHostToDevice));
   p.thr_key = d_thr.as<unsigned long long>();
   p.rec_count = d_count.as<unsigned int>();
@@ -762,9 +791,21 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   job->stats.timing_chunk_setup_ns =
       std::chrono::duration_cast<std::chrono::nanoseconds>(t_loop - t_start).count();
   uint64_t begin = 0;
+  uint32_t ramp = 0;
   while (begin < job->iterations) {
     if (job->cancel.load()) break;
-    const uint64_t end = std::min(job->iterations, begin + launch);
+    // without the wavefront filter the device records every candidate at or below the
+    // threshold as it stood before the launch, which is loose at first: such launches grow
+    // from 4096 candidates by 4x up to the launch size
+    const uint64_t size =
+        ramp_up ? std::min(launch, uint64_t(4096) << std::min(2 * ramp, 40u)) : launch;
+    const uint64_t end = std::min(job->iterations, begin + size);
+    if (p.systematic) {
+      const uint64_t lanes = uint64_t(blocks) * W * p.gen_group;
+      p.rounds = static_cast<uint32_t>(std::clamp<uint64_t>(size / (lanes * 2), 1, 1024));
+    } else {
+      p.rounds = 1;
+    }
     p.cand_begin = begin;
     p.cand_end = end;
     sink.bind(p);
@@ -775,12 +816,10 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     SQSG_HIP(launch_variant(p, job, variant, blocks, W, job->stream));
     SQSG_HIP(hipEventRecord(job->ev1, job->stream));
     unsigned int count = 0;
-    unsigned long long scored = 0, dk = 0;
+    unsigned long long scored = 0;
     SQSG_HIP(hipMemcpyAsync(&count, d_count.as<void>(), sizeof(count), hipMemcpyDeviceToHost,
                             job->stream));
     SQSG_HIP(hipMemcpyAsync(&scored, d_scored.as<void>(), sizeof(scored), hipMemcpyDeviceToHost,
-                            job->stream));
-    SQSG_HIP(hipMemcpyAsync(&dk, d_thr.as<void>(), sizeof(dk), hipMemcpyDeviceToHost,
                             job->stream));
     SQSG_HIP(hipStreamSynchronize(job->stream));
     float ms = 0;
@@ -793,12 +832,15 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     }
     if (scored != end - begin) throw DeviceError("device scored an unexpected candidate count");
     job->scored += scored;
-    sink.drain(job, count, p.nprob, job->stream);
-    // keep the device threshold at the tighter of its own bound and the exact host one
-    const unsigned long long hk = objective_key(job->vkeep);
-    if (hk < dk) SQSG_HIP(hipMemcpy(d_thr.as<void>(), &hk, sizeof(hk), hipMemcpyHostToDevice));
+    sink.drain(job, count, p.nprob, {{begin, end}}, job->stream);
+    const unsigned long long hk = search_key(job);
+    if (hk != thr) {
+      thr = hk;
+      SQSG_HIP(hipMemcpy(d_thr.as<void>(), &thr, sizeof(thr), hipMemcpyHostToDevice));
+    }
     job->stats.finished = job->scored;
     begin = end;
+    ++ramp;
     if (cb) {
       const auto now = clock::now();
       job->stats.timing_loop_ns =

@@ -9,6 +9,8 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "../../include/sqsgen_amd.h"
@@ -96,11 +98,39 @@ struct Problem {
   uint32_t row_stride() const { return (((N + 3) / 4) | 1u) * 4u; }
 };
 
-// Records of one objective level gathered from the device record sinks.
+// One objective level of the collection: the set of configurations inserted at that
+// objective (results.h:46-53), in candidate order.
 struct LevelRecords {
   std::vector<uint64_t> cand;
   std::vector<double> sub;        // one objective per problem and record
   std::vector<uint8_t> species;   // sorted-site packed species, `stride` bytes per record
+  std::unordered_multimap<uint64_t, uint32_t> seen;  // species hash -> record (Monte Carlo)
+  size_t size() const { return cand.size(); }
+};
+
+// Device records of one candidate range [begin, end), copied back to the host.
+struct HostRecords {
+  uint64_t begin = 0, end = 0;
+  std::vector<uint64_t> cand;
+  std::vector<double> obj, sub;
+  std::vector<uint8_t> species;
+};
+
+// The reference's result collection and search threshold, applied to the job's candidate
+// stream in candidate order (one thread's run of the worker loop, sqs.h:384-441): candidate
+// t is inserted when its objective is at or below `search`, the keep-th distinct objective
+// inserted before it (results.h:78-82; DBL_MAX while fewer exist, sqs.h:259-260), unless its
+// level already holds more than max_results_per_objective configurations (sqs.h:423-426);
+// levels are never evicted.  The device records a superset of the inserted candidates;
+// ranges that come back out of order (a record sink that overflowed runs again) wait in
+// `pending` until every earlier candidate has been merged.
+struct Collection {
+  std::map<double, LevelRecords> levels;
+  double search = 0;
+  uint64_t merged = 0;                       // candidates [0, merged) are merged
+  uint64_t stalled_block = ~uint64_t(0);     // see merge_in_order
+  std::map<uint64_t, HostRecords> pending;   // by range begin
+  void clear() { *this = Collection(); }
 };
 
 // Final, deduplicated results in ascending objective order (the sqs_result_pack levels).
@@ -136,8 +166,9 @@ struct sqsg_job {
   bool dense_ok = false;  // the dense MFMA contraction applies
   std::atomic<bool> cancel{false};
   // run state
-  std::map<double, sqsg::LevelRecords> levels;  // levels at or below the keep threshold
-  double vkeep = 0;
+  // blocks of the reference's thread-pool split of [0, iterations) (sqs.h:461-466)
+  uint64_t blk_size = 0, blk_split = 0;
+  sqsg::Collection coll;
   uint32_t stride = 0;        // packed species bytes per record (sum of problem sites)
   size_t cube_total = 0;      // doubles of one result's SRO cubes
   bool rank_fast = false;     // reported ranks fit 128-bit arithmetic
@@ -164,18 +195,28 @@ struct RecordSink {
   uint32_t cap = 0;
   void ensure(uint32_t n, uint32_t stride, uint32_t nprob);
   void bind(RunParams &p) const;
-  // copies `count` records back and merges them into the levels of the job
-  void drain(sqsg_job *job, uint32_t count, uint32_t nprob, hipStream_t s);
+  // copies `count` records of the candidate range [begin, end) back and merges them into
+  // the job's collection in candidate order
+  void drain(sqsg_job *job, uint32_t count, uint32_t nprob,
+             const std::vector<std::pair<uint64_t, uint64_t>> &ranges, hipStream_t s);
 };
 uint64_t default_launch_size(const sqsg_job *job);
+// the wavefront filter bounds the device records of the job (RunParams::wave_filter)
+bool records_bounded(const sqsg_job *job);
+// initial record sink size of a launch of `launch` candidates over `waves` wavefronts
+uint32_t first_sink_records(const sqsg_job *job, uint64_t waves, uint64_t launch);
 void begin_run(sqsg_job *job);
 // kernel variant a run uses (SQSG_KERNEL_SPARSE_WAVE / _LANE / _DENSE_MFMA)
 int resolve_kernel(const sqsg_job *job, int requested);
 hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, uint32_t blocks,
                           uint32_t waves_per_block, hipStream_t s);
 void reset_results(sqsg_job *job);
-void merge_records(sqsg_job *job, const uint64_t *cand, const double *obj, const double *sub,
-                   const uint8_t *species, uint32_t count);
+// hands the records of one candidate range to the collection (merged once every earlier
+// range is); returns the device threshold key for the next launch
+unsigned long long merge_records(sqsg_job *job, HostRecords &&recs);
+unsigned long long search_key(const sqsg_job *job);
+// block of the reference's thread-pool split that candidate t belongs to
+uint64_t block_of(const sqsg_job *job, uint64_t t);
 void finalize_results(sqsg_job *job);
 // accessors of finalized result r (original site order, atomic numbers)
 void result_species_z(const sqsg_job *job, uint64_t r, uint8_t *out);

@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void run_wave_kernel(RunParams p) {
               }
             } else {
               load_row(row, pr);
-              if (!(p.chunk_quirk && cand % p.chunk_size == 0))
+              if (!(p.chunk_quirk && block_start(p, cand)))
                 shuffle_candidate(Strided<0>{row}, pr, cand);
             }
           }
@@ -169,7 +169,23 @@ __global__ __launch_bounds__(256) void run_wave_kernel(RunParams p) {
         const unsigned long long key = okey(tot);
         const unsigned long long g =
             __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (key <= g) {
+        // the wave's L smallest distinct objectives of its earlier candidates: a candidate
+        // above the L-th has L smaller distinct objectives before it and is not inserted
+        const unsigned long long lk =
+            p.wave_filter && nlvl == p.L ? okey(lvl[p.L - 1]) : ~0ull;
+        if (p.wave_filter && key <= g) {
+          bool present = false;
+          for (uint32_t i = 0; i < nlvl; ++i) present |= (lvl[i] == tot);
+          if (!present && (nlvl < p.L || tot < lvl[nlvl - 1])) {
+            uint32_t pos = nlvl < p.L ? nlvl++ : nlvl - 1;
+            while (pos > 0 && lvl[pos - 1] > tot) {
+              lvl[pos] = lvl[pos - 1];
+              --pos;
+            }
+            lvl[pos] = tot;
+          }
+        }
+        if (key <= g && key <= lk) {
           uint32_t slot = 0;
           if (lane == 0) slot = atomicAdd(p.rec_count, 1u);
           slot = __shfl(slot, 0);
@@ -186,21 +202,6 @@ __global__ __launch_bounds__(256) void run_wave_kernel(RunParams p) {
               p.rec_cand[slot] = cand;
               p.rec_obj[slot] = tot;
               for (uint32_t q = 0; q < p.nprob; ++q) p.rec_sub[slot * p.nprob + q] = sub[q];
-            }
-          }
-          // the wave's L smallest distinct objectives; its L-th bounds the global one
-          bool present = false;
-          for (uint32_t i = 0; i < nlvl; ++i) present |= (lvl[i] == tot);
-          if (!present && (nlvl < p.L || tot < lvl[nlvl - 1])) {
-            uint32_t pos = nlvl < p.L ? nlvl++ : nlvl - 1;
-            while (pos > 0 && lvl[pos - 1] > tot) {
-              lvl[pos] = lvl[pos - 1];
-              --pos;
-            }
-            lvl[pos] = tot;
-            if (nlvl == p.L && lane == 0) {
-              const unsigned long long k = okey(lvl[p.L - 1]);
-              if (k < g) atomicMin(p.thr_key, k);
             }
           }
         }

@@ -153,7 +153,7 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
         }
       } else {
         load_col(col, pr);
-        if (active && !(p.chunk_quirk && cand % p.chunk_size == 0))
+        if (active && !(p.chunk_quirk && block_start(p, cand)))
           shuffle_candidate(Strided<6>{col}, pr, cand);
       }
     }
@@ -171,12 +171,12 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
     scored += static_cast<unsigned long long>(__popcll(__ballot(active)));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // the wave's own L-th distinct level bounds the final levels as well: a candidate above
-    // it has L better distinct objectives and cannot be kept, so it is not recorded
+    // a candidate above the threshold or above the filter's L-th level (as it stood before
+    // this round) cannot be inserted
+    const unsigned long long lk = filter_bound(lf, p);
     const bool below = active && okey(tot) <= g;
-    offer_levels(lf, p, __ballot(below), tot, g, lane);
-    const unsigned long long lk = lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
     const bool pass = below && okey(tot) <= lk;
+    offer_levels(lf, p, __ballot(below), tot);
     if (pass) {
       const uint32_t slot = atomicAdd(p.rec_count, 1u);
       if (slot < p.rec_cap) {

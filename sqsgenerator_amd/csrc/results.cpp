@@ -1,22 +1,26 @@
-// Result collection on the host: the deterministic part of sqs_result_collection
-// (core/results.h:42-99).  The device appends every candidate at or below its running
-// threshold; the host keeps the keep + 1 best distinct objective levels with all their
-// ties, deduplicates configurations per level (one set per objective, results.h:46-53),
-// applies max_results_per_objective (the reference inserts while the level holds at most
-// `max` entries, sqs.h:423-426), and maps results back to the original site order and
-// atomic numbers (postprocess_results, sqs.h:94-122) with the reported rank of
-// sqs_result_wrapper::rank (results.h:163-169).
+// Result collection on the host: sqs_result_collection (core/results.h:42-99) with the
+// optimizer's search threshold (sqs.h:199-206,421-438), applied to the job's candidate stream
+// in candidate order, which is the order one thread of the reference walks its blocks in.
+// The device records every candidate at or below the threshold as it stood before the launch
+// (a superset of what the reference inserts, since the threshold only decreases); the host
+// replays those records in candidate order with the reference's rules: insert at or below
+// the keep-th distinct objective inserted so far, one set of configurations per objective,
+// at most max_results_per_objective + 1 per level, never evict.  Results are then mapped back
+// to the original site order and atomic numbers (postprocess_results, sqs.h:94-122) with the
+// reported rank of sqs_result_wrapper::rank (results.h:163-169).
 //
 // Levels can hold tens of millions of ties (the 32-site A16B16 exhaustive search has
 // 17,271,296 optimal configurations), so records are stored as flat per-level arrays,
 // deduplicated by hashing and ranked with 128-bit arithmetic on all host cores.
 #include <algorithm>
+#include <cfloat>
 #include <cstring>
 #include <limits>
 #include <thread>
 #include <unordered_map>
 
 #include "job.hpp"
+#include "kernels.hpp"
 #include "permutation.hpp"
 
 namespace sqsg {
@@ -66,71 +70,110 @@ template <class Fn> void parallel_for(uint64_t n, Fn &&fn) {
 }  // namespace
 
 void reset_results(sqsg_job *job) {
-  job->levels.clear();
+  job->coll.clear();
+  job->coll.search = DBL_MAX;  // _search_objective starts at numeric_limits<T>::max()
   job->res.clear();
-  job->vkeep = std::numeric_limits<double>::infinity();
 }
 
-void merge_records(sqsg_job *job, const uint64_t *cand, const double *obj, const double *sub,
-                   const uint8_t *species, uint32_t count) {
+uint64_t block_of(const sqsg_job *job, uint64_t t) {
+  if (t < job->blk_split) return t / (job->blk_size + 1);
+  const uint64_t wide = job->blk_split / (job->blk_size + 1);
+  return wide + (t - job->blk_split) / job->blk_size;
+}
+
+unsigned long long search_key(const sqsg_job *job) { return objective_key(job->coll.search); }
+
+namespace {
+
+// One range's records in candidate order (the worker loop of sqs.h:384-441 on one thread).
+// In interact mode a candidate skipped for max_results_per_objective also skips the shuffle
+// or the lexicographic successor that follows it (the `continue` at sqs.h:425), so the rest
+// of its block re-scores that same configuration, which is skipped again: nothing else of
+// that block is inserted.
+void merge_in_order(sqsg_job *job, const HostRecords &r) {
+  Collection &C = job->coll;
   const uint32_t nprob = static_cast<uint32_t>(job->problems.size());
   const uint32_t stride = job->stride;
-  const size_t L = job->keep + 1;
-  for (uint32_t k = 0; k < count; ++k) {
-    if (obj[k] > job->vkeep) continue;
-    auto &lv = job->levels[obj[k]];
-    lv.cand.push_back(cand[k]);
-    lv.sub.insert(lv.sub.end(), sub + size_t(k) * nprob, sub + size_t(k + 1) * nprob);
-    lv.species.insert(lv.species.end(), species + size_t(k) * stride,
-                      species + size_t(k + 1) * stride);
-    if (job->levels.size() > L) {
-      job->levels.erase(std::prev(job->levels.end()));
-      job->vkeep = std::prev(job->levels.end())->first;
-    } else if (job->levels.size() == L) {
-      job->vkeep = std::prev(job->levels.end())->first;
+  const bool systematic = job->iteration_mode == SQSG_ITERATION_SYSTEMATIC;
+  const bool stalls = job->sublattice_mode == SQSG_SUBLATTICE_INTERACT;
+  const size_t n = r.cand.size();
+  std::vector<uint32_t> order(n);
+  for (size_t k = 0; k < n; ++k) order[k] = static_cast<uint32_t>(k);
+  std::sort(order.begin(), order.end(),
+            [&](uint32_t a, uint32_t b) { return r.cand[a] < r.cand[b]; });
+  for (uint32_t k : order) {
+    const uint64_t t = r.cand[k];
+    const double o = r.obj[k];
+    if (!(o <= C.search)) continue;
+    if (job->max_results >= 0 && C.stalled_block != ~uint64_t(0) &&
+        block_of(job, t) == C.stalled_block)
+      continue;
+    auto it = C.levels.find(o);
+    if (job->max_results >= 0 && it != C.levels.end() &&
+        it->second.size() > static_cast<size_t>(job->max_results)) {
+      if (stalls) C.stalled_block = block_of(job, t);
+      continue;
+    }
+    const uint8_t *sp = r.species.data() + size_t(k) * stride;
+    const bool fresh = it == C.levels.end();
+    if (fresh) it = C.levels.emplace(o, LevelRecords{}).first;
+    LevelRecords &lv = it->second;
+    if (!systematic) {  // a configuration repeats only in Monte Carlo runs
+      const uint64_t h = hash_bytes(sp, stride);
+      bool dup = false;
+      for (auto [a, b] = lv.seen.equal_range(h); a != b && !dup; ++a)
+        dup = std::memcmp(lv.species.data() + size_t(a->second) * stride, sp, stride) == 0;
+      if (dup) continue;
+      lv.seen.emplace(h, static_cast<uint32_t>(lv.size()));
+    }
+    lv.cand.push_back(t);
+    lv.sub.insert(lv.sub.end(), r.sub.begin() + size_t(k) * nprob,
+                  r.sub.begin() + size_t(k + 1) * nprob);
+    lv.species.insert(lv.species.end(), sp, sp + stride);
+    if (o < job->stats.best_objective) {
+      job->stats.best_objective = o;
+      job->stats.best_rank = t;
+    }
+    // update_objectives({objective, nth_best(keep)}) (sqs.h:435, 203-206)
+    if (fresh && C.levels.size() > job->keep) {
+      const double nb = std::next(C.levels.begin(), static_cast<long>(job->keep))->first;
+      if (nb < C.search) C.search = nb;
     }
   }
+}
+
+}  // namespace
+
+unsigned long long merge_records(sqsg_job *job, HostRecords &&recs) {
+  Collection &C = job->coll;
+  if (recs.begin < C.merged) throw std::logic_error("candidate range merged twice");
+  const uint64_t b = recs.begin;
+  C.pending.emplace(b, std::move(recs));
+  while (!C.pending.empty() && C.pending.begin()->first == C.merged) {
+    HostRecords &r = C.pending.begin()->second;
+    merge_in_order(job, r);
+    C.merged = r.end;
+    C.pending.erase(C.pending.begin());
+  }
+  return search_key(job);
 }
 
 void finalize_results(sqsg_job *job) {
   ResultSet &R = job->res;
   R.clear();
-  const uint32_t nprob = static_cast<uint32_t>(job->problems.size());
-  const uint32_t stride = job->stride;
-  const bool systematic = job->iteration_mode == SQSG_ITERATION_SYSTEMATIC;
+  if (!job->coll.pending.empty())
+    throw std::logic_error("records of a candidate range were never merged");
   R.level_start.push_back(0);
-  for (auto &[obj, lv] : job->levels) {
-    const size_t n = lv.cand.size();
-    std::vector<uint32_t> order(n);
-    for (size_t k = 0; k < n; ++k) order[k] = static_cast<uint32_t>(k);
-    std::sort(order.begin(), order.end(),
-              [&](uint32_t a, uint32_t b) { return lv.cand[a] < lv.cand[b]; });
-    // a candidate index is written by exactly one record; configurations repeat only in
-    // MC (the same permutation drawn twice), never in systematic enumeration
-    std::unordered_map<uint64_t, std::vector<uint32_t>> seen;
-    if (!systematic) seen.reserve(n * 2);
-    size_t kept = 0;
-    for (uint32_t k : order) {
-      if (job->max_results >= 0 && kept > static_cast<size_t>(job->max_results)) break;
-      const uint8_t *sp = lv.species.data() + size_t(k) * stride;
-      if (!systematic) {
-        auto &bucket = seen[hash_bytes(sp, stride)];
-        bool dup = false;
-        for (uint32_t o : bucket)
-          dup |= std::memcmp(lv.species.data() + size_t(o) * stride, sp, stride) == 0;
-        if (dup) continue;
-        bucket.push_back(k);
-      }
-      R.cand.push_back(lv.cand[k]);
-      R.sub.insert(R.sub.end(), lv.sub.begin() + size_t(k) * nprob,
-                   lv.sub.begin() + size_t(k + 1) * nprob);
-      R.species.insert(R.species.end(), sp, sp + stride);
-      ++kept;
-    }
+  for (auto &[obj, lv] : job->coll.levels) {
+    R.cand.insert(R.cand.end(), lv.cand.begin(), lv.cand.end());
+    R.sub.insert(R.sub.end(), lv.sub.begin(), lv.sub.end());
+    R.species.insert(R.species.end(), lv.species.begin(), lv.species.end());
     R.level_obj.push_back(obj);
     R.level_start.push_back(R.cand.size());
+    lv = LevelRecords{};  // release as we go: packs can hold tens of millions of ties
   }
-  job->levels.clear();
+  job->coll.clear();
+  job->coll.search = DBL_MAX;
   // SRO cubes are re-scored on the device: eagerly for packs of ordinary size
   if (R.size() && R.size() <= (uint64_t(1) << 20)) {
     std::vector<double> sro(R.size() * job->cube_total);

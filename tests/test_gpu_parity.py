@@ -10,43 +10,11 @@ import pytest
 import workloads as W
 from oracle import oracle as O
 import sqsgenerator_amd as S
+from gpu_oracle import Rescorer, assert_packs_equal, device_pack, oracle_inputs, oracle_pack
 from sqsgenerator_amd.core import Job, parse_config
 
 pytestmark = pytest.mark.gpu
 REL_TOL = 1e-12
-
-
-def oracle_inputs(cfg, job, p=0):
-    """Problem p rebuilt by the oracle from the configuration; the only input taken from the
-    job is the site sort order (libstdc++ std::sort of equal keys, not pinned by the
-    reference)."""
-    info = job.problem(p)
-    st = cfg.structure()
-    applied = np.array(st.species, dtype=np.uint8)
-    for sl in cfg.composition:
-        it = iter(sl.sites)
-        for z, n in sl.composition.items():
-            for _ in range(n):
-                applied[next(it)] = z
-    if cfg.sublattice_mode == S.split:
-        sites = np.array(cfg.composition[p].sites)
-    else:
-        sites = np.arange(len(applied))
-    order = info["sort_order"]
-    frac = st.frac_coords[sites][order]
-    z = applied[sites][order]
-    dm = O.distance_matrix(st.lattice, frac)
-    shells = sorted(cfg.shell_weights[p])
-    sm = O.shell_matrix(dm, cfg.shell_radii[p])
-    pairs = O.pairs(sm, shells)
-    zs = sorted(set(z.tolist()))
-    packed = np.array([zs.index(v) for v in z], dtype=np.uint8)
-    sm_unsorted = O.shell_matrix(O.distance_matrix(st.lattice, st.frac_coords[sites]),
-                                 cfg.shell_radii[p])
-    pref = O.compute_prefactors(sm_unsorted, shells, applied[sites])
-    pw = O.scaled_pair_weights(cfg.pair_weights[p], [cfg.shell_weights[p][s] for s in shells])
-    return dict(pairs=pairs, packed=packed, pref=pref, pw=pw, tgt=cfg.target_objective[p],
-                S=len(shells), K=len(zs), bounds=info["bounds"])
 
 
 def random_candidates(packed, bounds, B, rng):
@@ -65,6 +33,8 @@ WORKLOADS = {
     "rocksalt216_split": lambda: W.rocksalt_split_216(),
     "fcc256_quinary": lambda: W.quinary_256(),
     "fcc864_quinary": lambda: W.quinary_864(),
+    "fcc64_ternary": lambda: W.ternary_64(),
+    "fcc500_quaternary": lambda: W.quaternary_500(),
 }
 
 
@@ -160,40 +130,24 @@ def test_systematic_partial_sublattice(gpu):
             assert sorted(z for z in sol.species if z != 2) == [13] * 8 + [22] * 8
 
 
-def _rescore(cfg, sol):
-    """oracle re-score of a reported (original order, Z) configuration"""
-    st = cfg.structure()
-    shells = sorted(cfg.shell_weights[0])
-    sm = O.shell_matrix(O.distance_matrix(st.lattice, st.frac_coords), cfg.shell_radii[0])
-    pairs = O.pairs(sm, shells)
-    zs = sorted(set(sol.species))
-    packed = np.array([zs.index(v) for v in sol.species], dtype=np.uint8)
-    bonds = O.count_bonds(pairs, packed, len(shells), len(zs))
-    pw = O.scaled_pair_weights(cfg.pair_weights[0], [cfg.shell_weights[0][s] for s in shells])
-    return O.compute_objective(bonds, cfg.prefactors[0], pw, cfg.target_objective[0])
-
-
 def test_mc_plain_32_rescored(gpu):
-    """every MC-reported structure is re-scored by the CPU oracle; ranks by the oracle"""
+    """configuration (1): 1e6 MC candidates; the pack equals the oracle driver's over the
+    same candidate stream, and every reported structure is re-scored from the configuration
+    (objective bits, SRO, rank)"""
     cfg = parse_config(W.plain_mc_32(iterations=1_000_000))
-    res = S.optimize(cfg)
+    job = Job(cfg)
+    job.run()
+    res = job.results()
     assert res.statistics["finished"] == 1_000_000
-    n = 0
-    for obj, rset in res:
-        for sol in rset:
-            o, sro = _rescore(cfg, sol)
-            assert np.float64(o).view(np.uint64) == np.float64(sol.objective).view(np.uint64)
-            assert np.array_equal(sro, sol.sro())
-            zs = sorted(set(sol.species))
-            assert int(sol.rank()) == O.rank_permutation([zs.index(v) for v in sol.species])
-            n += 1
-    assert n >= 1
-    assert len(res) <= cfg.keep + 1
+    assert Rescorer(cfg).check_pack(res) >= 1
+    assert_packs_equal(device_pack(res), oracle_pack(cfg, job))
+    job.close()
 
 
 def test_mc_tie_set_complete_16(gpu):
     """MC over the 12870-candidate space finds every configuration of the best two levels
-    (1e6 draws: miss probability per configuration e^-77)"""
+    (1e6 draws: miss probability per configuration e^-77); every candidate at or below the
+    final threshold is inserted, so the pack's two best levels are complete"""
     base = W.reference_test_16()
     del base["pair_weights"]
     sys_cfg = parse_config(base)
@@ -214,13 +168,11 @@ def test_mc_tie_set_complete_16(gpu):
     ref_levels = {}
     for o, sp, _ in ref:
         ref_levels.setdefault(o, set()).add(tuple(np.where(sp == 1, 22, 13).tolist()))
-    # the reference never evicts levels inserted early; the deterministic part of its pack
-    # is the keep + 1 best levels, which is what the device path reports
-    ref_levels = {o: ref_levels[o] for o in sorted(ref_levels)[:mc_cfg.keep + 1]}
+    best = sorted(ref_levels)[:mc_cfg.keep + 1]
     res = S.optimize(mc_cfg)
     got = {o: {tuple(s.species) for s in rs} for o, rs in res}
-    assert sorted(got) == sorted(ref_levels)
-    for o in got:
+    assert sorted(got)[:mc_cfg.keep + 1] == best
+    for o in best:
         assert got[o] == ref_levels[o], f"level {o}"
 
 
@@ -248,6 +200,10 @@ def test_variants_agree_mc(gpu, name):
         job.close()
     for other in packs[1:]:
         assert other == packs[0]
+    job = Job(parse_config(d))
+    job.run()
+    assert Rescorer(job.config).check_pack(job.results()) > 0
+    job.close()
 
 
 def test_variants_agree_systematic(gpu):
@@ -322,6 +278,7 @@ def test_batch_matches_single_job_runs(gpu, use_graph):
     for j, ref in zip(jobs, alone):
         assert j.timing()[2] == 131072
         assert _pack_signature(j.results()) == ref
+        assert Rescorer(j.config).check_pack(j.results()) > 0
         j.close()
 
 
@@ -340,8 +297,9 @@ def test_server_results_alone_and_in_the_mix(gpu):
     """continuous batching: jobs admitted while others are resident produce the same packs
     as when run alone"""
     names = ["fcc256_quinary", "fcc32_3shells", "hcp128_binary", "fcc108_ternary",
-             "rocksalt216_split", "bcc128_quaternary"]
-    its = [65536, 200000, 100000, 70000, 50000, 90000]
+             "rocksalt216_split", "bcc128_quaternary", "fcc64_ternary", "fcc500_quaternary",
+             "fcc864_quinary"]
+    its = [65536, 200000, 100000, 70000, 50000, 90000, 120000, 30000, 12000]
     alone = []
     for name, it in zip(names, its):
         job = Job(parse_config(_small(name, it)))
@@ -356,12 +314,15 @@ def test_server_results_alone_and_in_the_mix(gpu):
     tickets += [srv.submit(j) for j in jobs[2:4]]
     srv.step()
     srv.step()
-    tickets += [srv.submit(j) for j in jobs[4:]]
+    tickets += [srv.submit(j) for j in jobs[4:7]]
+    srv.step()
+    tickets += [srv.submit(j) for j in jobs[7:]]
     srv.drain()
     for t, j, ref in zip(tickets, jobs, alone):
         st = srv.poll(t)
         assert st["done"] and st["t_admit"] >= st["t_submit"] and st["t_done"] >= st["t_admit"]
         assert _pack_signature(j.results()) == ref
+        assert Rescorer(j.config).check_pack(j.results()) > 0
     srv.close()
 
 
@@ -418,6 +379,7 @@ def test_batch_dense_matches_single_job_runs(gpu, use_graph):
     for j, ref in zip(jobs, alone):
         assert j.timing()[2] == 131072
         assert _pack_signature(j.results()) == ref
+        assert Rescorer(j.config).check_pack(j.results()) > 0
         j.close()
 
 

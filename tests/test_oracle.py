@@ -355,3 +355,117 @@ def test_max_results_per_objective():
     prob = _reference_test_16_problem(pair_weights=0)
     st, res = O.run([prob], 12870, systematic=True, max_results_per_objective=5, threads=1)
     assert len(res) == 6
+
+
+# ---- the collection rule on an explicit candidate stream ---------------------------------
+# A plain-Python statement of the worker loop on one thread (sqs.h:384-441, 461-466) that the
+# oracle's driver must reproduce, in both its candidate modes.
+
+SECRET0 = 0x2d358dccaa6c78a5  # rapidhash.h:129-131
+
+
+def _block_of(iterations, chunk):
+    """BS::thread_pool's detach_blocks(0, iterations, f, iterations / chunk): near-equal
+    blocks, the first `iterations % blocks` one larger"""
+    nb = max(1, iterations // chunk)
+    size, rem = divmod(iterations, nb)
+    out, t = [], 0
+    for b in range(nb):
+        n = size + (1 if b < rem else 0)
+        out += [b] * n
+        t += n
+    assert t == iterations
+    return out
+
+
+def _collect(stream, blocks, keep, max_results, interact=True):
+    """stream: (configuration bytes, objective) per candidate in index order"""
+    levels, search, stalled = {}, np.finfo(np.float64).max, None
+    for (conf, o), b in zip(stream, blocks):
+        if b == stalled or not (o <= search):
+            continue
+        if max_results is not None and o in levels and len(levels[o]) > max_results:
+            if interact:  # the skipped configuration is scored again until the block ends
+                stalled = b
+            continue
+        lv = levels.setdefault(o, [])
+        if conf not in lv:
+            lv.append(conf)
+        distinct = sorted(levels)
+        if len(distinct) > keep:
+            search = min(search, distinct[keep])
+    return {o: set(v) for o, v in levels.items()}
+
+
+def _score(prob, c):
+    return O.compute_objective(O.count_bonds(prob.pairs, c, prob.S, prob.K), prob.pref, prob.pw,
+                               prob.tgt)[0]
+
+
+def _counter_stream(prob, iterations, chunk, seed):
+    """the device path's Monte Carlo candidates: candidate t is the sorted configuration
+    shuffled with the draws [t D, (t + 1) D) of the seed's stream, the first candidate of an
+    interact block unshuffled"""
+    D = sum(int(hi) - int(lo) - 1 for lo, hi in np.asarray(prob.bounds).reshape(-1, 2))
+    blocks = _block_of(iterations, chunk)
+    out = []
+    for t in range(iterations):
+        c = prob.species.copy()
+        if t and blocks[t] == blocks[t - 1]:
+            c, _ = O.shuffle_random(c, prob.bounds, (seed + t * D * SECRET0) % (1 << 64))
+        out.append((c.tobytes(), _score(prob, c)))
+    return out, blocks
+
+
+def _pack(res):
+    out = {}
+    for o, sp, _ in res:
+        out.setdefault(o, set()).add(sp.tobytes())
+    return out
+
+
+def test_block_split_is_near_equal():
+    assert [_block_of(5001, 700).count(b) for b in range(7)] == [715] * 3 + [714] * 4
+    assert _block_of(10, 10) == [0] * 10
+    assert [_block_of(12870, 1000).count(b) for b in range(12)] == [1073] * 6 + [1072] * 6
+
+
+def test_counter_stream_holds_every_candidate():
+    """keep above the candidate count inserts every distinct configuration: the oracle's
+    counter-stream pack is exactly the set of candidates built from the golden-pinned
+    shuffle and draw stream (block starts unshuffled)"""
+    prob = _reference_test_16_problem()
+    prob.seed = 77
+    stream, _ = _counter_stream(prob, 2001, 300, 77)
+    _, res = O.run([prob], 2001, chunk_size=300, keep=10 ** 9, counter_stream=True)
+    got = _pack(res)
+    want = {}
+    for conf, o in stream:
+        want.setdefault(o, set()).add(conf)
+    assert got == want
+
+
+@pytest.mark.parametrize("max_results", [None, 0, 1])
+@pytest.mark.parametrize("keep", [1, 2])
+def test_counter_stream_collection_rule(keep, max_results):
+    prob = _reference_test_16_problem()
+    prob.seed = 5
+    stream, blocks = _counter_stream(prob, 1501, 400, 5)
+    _, res = O.run([prob], 1501, chunk_size=400, keep=keep, max_results_per_objective=max_results,
+                   counter_stream=True)
+    assert _pack(res) == _collect(stream, blocks, keep, max_results)
+
+
+@pytest.mark.parametrize("max_results", [None, 0, 2])
+def test_systematic_collection_rule(max_results):
+    """systematic blocks: rank order, the stall after a max_results skip included"""
+    prob = _reference_test_16_problem()
+    conf = np.array(prob.species)
+    stream = []
+    for r in range(1, 12871):
+        c = O.unrank_permutation(conf, r)
+        stream.append((c.tobytes(), _score(prob, c)))
+    blocks = _block_of(12870, 1000)
+    _, res = O.run([prob], 12870, chunk_size=1000, systematic=True, keep=3,
+                   max_results_per_objective=max_results)
+    assert _pack(res) == _collect(stream, blocks, 3, max_results)

@@ -59,6 +59,18 @@ def quinary_864(iterations=1 << 20, seed=7):
                iterations=iterations, seed=seed, thread_config=1)
 
 
+def ternary_64(iterations=1_000_000, seed=1005):
+    """FCC 2x2x4 (64 sites), Al 22 / Ti 21 / Mo 21, 4 shells (a serving-mix cell size)"""
+    return fcc(4.05, (2, 2, 4), {"Al": 22, "Ti": 21, "Mo": 21}, 4, iterations=iterations,
+               seed=seed, thread_config=1)
+
+
+def quaternary_500(iterations=1_000_000, seed=1006):
+    """FCC 5x5x5 (500 sites), Fe/Co/Ni/Cr 125 each, 6 shells (a serving-mix cell size)"""
+    return fcc(4.05, (5, 5, 5), {"Fe": 125, "Co": 125, "Ni": 125, "Cr": 125}, 6,
+               iterations=iterations, seed=seed, thread_config=1)
+
+
 def ternary_108(iterations=10_000_000, seed=1001):
     return fcc(4.05, (3, 3, 3), {"Al": 36, "Ti": 36, "Mo": 36}, 3, iterations=iterations,
                seed=seed, thread_config=1)
