@@ -62,6 +62,25 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def cpu_share() -> dict:
+    """Host threads the reference's default would use (std::thread::hardware_concurrency(),
+    io/config/combined.h:105-107) and the cores this process may actually run on: its CPU
+    affinity and the cgroup's CPU quota (threads beyond those only time-share the same
+    cores)"""
+    hw = os.cpu_count() or 1
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    usable = min(hw, affinity, int(quota) if quota else hw)
+    return {"hardware_concurrency": hw, "affinity": affinity, "cgroup_cpu_quota": quota,
+            "threads": max(1, usable)}
+
+
 def cpu_baseline(job, budget_s: float) -> dict:
     """The oracle's restatement of the reference worker loop (sqs.h:348-455), multithreaded
     with the shared RNG state, on a bounded sample of the same workload."""
@@ -72,7 +91,8 @@ def cpu_baseline(job, budget_s: float) -> dict:
     prob = O.Problem(info["pairs"].astype(np.uint64), info["species_packed"],
                      np.array(info["bounds"]).ravel(), info["prefactors"], info["pair_weights"],
                      info["target"], seed=7)
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    share = cpu_share()
+    threads = share["threads"]
     st, _ = O.run([prob], 2000, chunk_size=2000, threads=1, collect=False)
     rate1 = st.finished / max(st.seconds, 1e-9)
     # one thread, about a tenth of the budget
@@ -86,7 +106,7 @@ def cpu_baseline(job, budget_s: float) -> dict:
     s3, _ = O.run([prob], n3, chunk_size=max(1000, n3 // (threads * 8)), threads=threads,
                   shared_rng=False, collect=False)
     return {"value": st.finished / st.seconds, "unit": "candidates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": dict(share, cpu_model=cpu_model()),
             "sample": f"{st.finished} MC candidates of the same 256-site job, {threads} threads, "
                       f"shared RNG state, {st.seconds:.1f} s on {cpu_model()}",
             "one_thread": {"value": s1.finished / s1.seconds, "cores": 1,

@@ -640,6 +640,8 @@ class Job:
     """Owning handle of an sqsg_job (one optimisation on the device)."""
 
     def __init__(self, cfg: SqsConfiguration):
+        if isinstance(cfg, ParseError):
+            raise ValueError(f"{cfg.key}: {cfg.msg}")
         self.config = cfg
         self._arrays = _Arrays(cfg)
         h = C.c_void_p()

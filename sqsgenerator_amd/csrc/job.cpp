@@ -219,11 +219,13 @@ void Problem::build_lane(uint32_t field_bits) {
 
 // Tiles in the A-operand layout of v_mfma_i32_32x32x32_i8: lane l, byte q of tile (s, I, J)
 // is U_s[32I + perm(l % 32)][32J + 16(l / 32) + q] with perm(m) = 16((m / 4) % 2) +
-// 4(m / 8) + m % 4 (the accumulator row order of the 32 x 32 result).  Tiles are stored
-// shell, then I, then J >= I, zero tiles included: for kernel tile bounds up to 8 the whole
-// upper grid of the bound TT (tiles past the problem's T tiles are zeros, so an offset is a
-// function of the shell and the grid position alone), for larger bounds the I <= J < T
-// tiles (the kernel checks the bounds per position).
+// 4(m / 8) + m % 4 (the accumulator row order of the 32 x 32 result).  Layout invariant
+// the kernels rely on (their tile loads take the offset in the buffer instruction's scalar
+// operand, which is not range-checked): for every shell, and for one zero shell after the
+// last, the whole upper grid I <= J < TT of the kernel's tile bound TT, in (shell, I, J)
+// order, zero tiles included (tiles past the problem's T tiles are zeros), so a tile's
+// offset is a function of the shell and the grid position alone and a prefetch of up to
+// one shell past the end stays inside the stream: dtiles_n = (S + 1) TT (TT + 1) / 2.
 void Problem::build_dense() {
   if (d_dtiles.size()) return;
   dT = (N + 31) / 32;
@@ -260,6 +262,8 @@ void Problem::build_dense() {
   // inside the stream (those loads take their offset unchecked)
   tiles.resize(tiles.size() + size_t(TT) * (TT + 1) / 2 * 1024, 0);
   dtiles_n = static_cast<uint32_t>(tiles.size() / 1024);
+  if (size_t(dtiles_n) != size_t(S + 1) * TT * (TT + 1) / 2)
+    throw std::logic_error("dense tile stream does not hold the whole grid of every shell");
   std::vector<uint8_t> sp(n32, 0xff);
   std::copy(species_packed.begin(), species_packed.end(), sp.begin());
   upload(d_dtiles, tiles, 16);
@@ -575,8 +579,10 @@ bool records_bounded(const sqsg_job *job) {
 
 uint32_t first_sink_records(const sqsg_job *job, uint64_t waves, uint64_t launch) {
   // with the filter, a wavefront records about L ln(n) + ties of its n candidates while the
-  // threshold is still loose
-  const uint64_t want = std::max<uint64_t>(uint64_t(1) << 14, waves * (job->keep + 1) * 16);
+  // threshold is still loose; without it, launches start short and the sink grows on demand
+  const uint64_t want = records_bounded(job)
+                            ? std::max<uint64_t>(uint64_t(1) << 14, waves * (job->keep + 1) * 16)
+                            : uint64_t(1) << 16;
   return static_cast<uint32_t>(std::min<uint64_t>({launch, want, uint64_t(1) << 22}));
 }
 
