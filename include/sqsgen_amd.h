@@ -188,6 +188,15 @@ SQSG_API int sqsg_job_set_launch_size(sqsg_job *job, uint64_t candidates_per_lau
  * (SQSG_ERR_INVALID_ARGUMENT) when a pair weight is negative; other kernels score in full. */
 SQSG_API int sqsg_job_set_pruning(sqsg_job *job, int enable);
 SQSG_API int sqsg_run(sqsg_job *job, sqsg_progress_fn cb, void *user);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc): sqsg_run without device allocations.  The run's
+ * threshold, counters and record sink live in a 16 MiB device array of the library (one per
+ * GPU, held until the process ends); a launch whose records exceed it runs again as shorter
+ * launches, and a job whose single record exceeds it fails with SQSG_ERR_OUT_OF_RANGE.  The
+ * job's inputs must already be resident (a first sqsg_run of the job uploads them), else
+ * SQSG_ERR_INVALID_ARGUMENT.  SRO cubes are computed when first read, not by the run.  Calls
+ * share the array: within a process they are serialised; the results equal sqsg_run's. */
+SQSG_API int sqsg_run_noalloc(sqsg_job *job, sqsg_progress_fn cb, void *user);
 /* cooperative cancellation, async-safe (sqs.h:385-396) */
 SQSG_API void sqsg_cancel(sqsg_job *job);
 SQSG_API int sqsg_job_statistics(sqsg_job *job, sqsg_statistics *stats);

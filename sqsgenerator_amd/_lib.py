@@ -115,6 +115,7 @@ SIGNATURES = {
     "sqsg_job_set_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "sqsg_job_set_launch_size": (C.c_int, [C.c_void_p, C.c_uint64]),
     "sqsg_run": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
+    "sqsg_run_noalloc": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
     "sqsg_cancel": (None, [C.c_void_p]),
     "sqsg_job_statistics": (C.c_int, [C.c_void_p, C.POINTER(Statistics)]),
     "sqsg_job_timing": (C.c_int, [C.c_void_p, f64p, u64p, u64p]),

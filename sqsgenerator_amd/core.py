@@ -730,6 +730,11 @@ class Job:
         if raised:
             raise raised[0]
 
+    def run_noalloc(self) -> None:
+        """the same run without device allocations (sqsg_run_noalloc): the job's inputs must
+        be resident already, i.e. after a first run()"""
+        L.check(L.lib().sqsg_run_noalloc(self.handle, L.PROGRESS_FN(), None))
+
     def cancel(self) -> None:
         L.lib().sqsg_cancel(self.handle)
 

@@ -63,6 +63,10 @@ sqsg::Lattice lattice(const double *m) {
 
 }  // namespace
 
+namespace sqsg {
+int guarded(const std::function<void()> &fn) { return guard(fn); }
+}  // namespace sqsg
+
 extern "C" {
 
 const char *sqsg_last_error(void) { return g_last_error.c_str(); }

@@ -702,20 +702,40 @@ hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, 
 
 void RecordSink::ensure(uint32_t n, uint32_t stride, uint32_t nprob) {
   // buffers are kept when large enough (a pooled sink may come from a job of another shape)
-  if (cand.size() < size_t(n) * sizeof(uint64_t)) cand.alloc(size_t(n) * sizeof(uint64_t));
-  if (obj.size() < size_t(n) * sizeof(double)) obj.alloc(size_t(n) * sizeof(double));
-  if (sub.size() < size_t(n) * nprob * sizeof(double))
-    sub.alloc(size_t(n) * nprob * sizeof(double));
-  if (species.size() < size_t(n) * stride) species.alloc(size_t(n) * stride);
+  if (cand_buf.size() < size_t(n) * sizeof(uint64_t)) cand_buf.alloc(size_t(n) * sizeof(uint64_t));
+  if (obj_buf.size() < size_t(n) * sizeof(double)) obj_buf.alloc(size_t(n) * sizeof(double));
+  if (sub_buf.size() < size_t(n) * nprob * sizeof(double))
+    sub_buf.alloc(size_t(n) * nprob * sizeof(double));
+  if (species_buf.size() < size_t(n) * stride) species_buf.alloc(size_t(n) * stride);
+  cand = cand_buf.as<uint64_t>();
+  obj = obj_buf.as<double>();
+  sub = sub_buf.as<double>();
+  species = species_buf.as<uint8_t>();
   cap = n;
+}
+
+uint32_t RecordSink::place(void *base, size_t bytes, uint32_t stride, uint32_t nprob) {
+  const size_t per = sizeof(uint64_t) + sizeof(double) * (1 + nprob) + stride;
+  const size_t n = bytes > 1024 ? (bytes - 1024) / per : 0;  // alignment slack of 4 arrays
+  cap = static_cast<uint32_t>(std::min<size_t>(n, UINT32_MAX));
+  auto at = [](uintptr_t x) { return (x + 255) & ~uintptr_t(255); };
+  uintptr_t o = at(reinterpret_cast<uintptr_t>(base));
+  cand = reinterpret_cast<uint64_t *>(o);
+  o = at(o + size_t(cap) * sizeof(uint64_t));
+  obj = reinterpret_cast<double *>(o);
+  o = at(o + size_t(cap) * sizeof(double));
+  sub = reinterpret_cast<double *>(o);
+  o = at(o + size_t(cap) * nprob * sizeof(double));
+  species = reinterpret_cast<uint8_t *>(o);
+  return cap;
 }
 
 void RecordSink::bind(RunParams &p) const {
   p.rec_cap = cap;
-  p.rec_cand = cand.as<uint64_t>();
-  p.rec_obj = obj.as<double>();
-  p.rec_sub = sub.as<double>();
-  p.rec_species = species.as<uint8_t>();
+  p.rec_cand = cand;
+  p.rec_obj = obj;
+  p.rec_sub = sub;
+  p.rec_species = species;
 }
 
 void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob,
@@ -726,13 +746,12 @@ void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob,
   std::vector<double> o(count), sb(size_t(count) * nprob);
   std::vector<uint8_t> sp(size_t(count) * stride);
   if (count) {
-    SQSG_HIP(hipMemcpyAsync(c.data(), cand.as<void>(), count * sizeof(uint64_t),
-                            hipMemcpyDeviceToHost, s));
-    SQSG_HIP(hipMemcpyAsync(o.data(), obj.as<void>(), count * sizeof(double),
-                            hipMemcpyDeviceToHost, s));
-    SQSG_HIP(hipMemcpyAsync(sb.data(), sub.as<void>(), sb.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, s));
-    SQSG_HIP(hipMemcpyAsync(sp.data(), species.as<void>(), sp.size(), hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipMemcpyAsync(c.data(), cand, count * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            s));
+    SQSG_HIP(hipMemcpyAsync(o.data(), obj, count * sizeof(double), hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipMemcpyAsync(sb.data(), sub, sb.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            s));
+    SQSG_HIP(hipMemcpyAsync(sp.data(), species, sp.size(), hipMemcpyDeviceToHost, s));
     SQSG_HIP(hipStreamSynchronize(s));
   }
   // one record set per candidate range (a step can hold a re-issued range and the next one)
@@ -770,12 +789,20 @@ void begin_run(sqsg_job *job) {
   job->scored = 0;
 }
 
-void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
+void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user, const FixedScratch *fixed) {
   using clock = std::chrono::steady_clock;
   const auto t_start = clock::now();
-  begin_run(job);
+  analyze_job(job);  // kernel applicability (host data), before the variant is resolved
   const int variant = resolve_kernel(job, job->kernel);
-  if (variant == SQSG_KERNEL_DENSE_MFMA || variant == SQSG_KERNEL_DENSE_PAIRED) ensure_dense(job);
+  const bool dense = variant == SQSG_KERNEL_DENSE_MFMA || variant == SQSG_KERNEL_DENSE_PAIRED;
+  if (fixed && !job->on_device)
+    throw std::invalid_argument("the job's inputs are not resident on the device yet (a first "
+                                "sqsg_run of the job uploads them)");
+  if (fixed && dense && !job->problems[0].d_dtiles.size())
+    throw std::invalid_argument("the job's dense tile stream is not resident on the device yet "
+                                "(a first sqsg_run of the job with this kernel uploads it)");
+  begin_run(job);
+  if (dense) ensure_dense(job);
   const uint32_t W = variant == SQSG_KERNEL_SPARSE_WAVE ? 4 : 1;
   uint32_t blocks = 0;
   RunParams p = make_run_params(job, variant, W, &blocks);
@@ -783,28 +810,38 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
   const uint32_t stride = job->stride;
   p.rec_stride = stride;
   RecordSink sink;
-  sink.ensure(first_sink_records(job, uint64_t(blocks) * W, launch), stride, p.nprob);
   const bool ramp_up = !records_bounded(job);
-  DevBuf d_thr(sizeof(unsigned long long)), d_count(sizeof(unsigned int)),
-      d_scored(sizeof(unsigned long long)), d_next(sizeof(unsigned int));
+  DevBuf d_ctl;
+  uint8_t *ctl = nullptr;  // threshold key, record count, scored count, segment counter
+  if (fixed) {
+    if (sink.place(static_cast<uint8_t *>(fixed->base) + 256, fixed->bytes - 256, stride,
+                   p.nprob) == 0)
+      throw std::out_of_range("a record does not fit the scratch buffer");
+    ctl = static_cast<uint8_t *>(fixed->base);
+  } else {
+    sink.ensure(first_sink_records(job, uint64_t(blocks) * W, launch), stride, p.nprob);
+    d_ctl.alloc(256);
+    ctl = d_ctl.as<uint8_t>();
+  }
   unsigned long long thr = search_key(job);
-  SQSG_HIP(hipMemcpy(d_thr.as<void>(), &thr, sizeof(thr), hipMemcpyHostToDevice));
-  p.thr_key = d_thr.as<unsigned long long>();
-  p.rec_count = d_count.as<unsigned int>();
-  p.scored = d_scored.as<unsigned long long>();
-  p.next_seg = d_next.as<unsigned int>();
+  SQSG_HIP(hipMemcpy(ctl, &thr, sizeof(thr), hipMemcpyHostToDevice));
+  p.thr_key = reinterpret_cast<unsigned long long *>(ctl);
+  p.rec_count = reinterpret_cast<unsigned int *>(ctl + 8);
+  p.scored = reinterpret_cast<unsigned long long *>(ctl + 16);
+  p.next_seg = reinterpret_cast<unsigned int *>(ctl + 24);
   const auto t_loop = clock::now();
   job->stats.timing_chunk_setup_ns =
       std::chrono::duration_cast<std::chrono::nanoseconds>(t_loop - t_start).count();
   uint64_t begin = 0;
   uint32_t ramp = 0;
+  uint64_t limit = launch;  // fixed scratch: launches shrink while their records overflow it
   while (begin < job->iterations) {
     if (job->cancel.load()) break;
     // without the wavefront filter the device records every candidate at or below the
     // threshold as it stood before the launch, which is loose at first: such launches grow
     // from 4096 candidates by 4x up to the launch size
-    const uint64_t size =
-        ramp_up ? std::min(launch, uint64_t(4096) << std::min(2 * ramp, 40u)) : launch;
+    const uint64_t size = std::min(
+        limit, ramp_up ? std::min(launch, uint64_t(4096) << std::min(2 * ramp, 40u)) : launch);
     const uint64_t end = std::min(job->iterations, begin + size);
     if (p.systematic) {
       const uint64_t lanes = uint64_t(blocks) * W * p.gen_group;
@@ -815,25 +852,29 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     p.cand_begin = begin;
     p.cand_end = end;
     sink.bind(p);
-    SQSG_HIP(hipMemsetAsync(d_count.as<void>(), 0, sizeof(unsigned int), job->stream));
-    SQSG_HIP(hipMemsetAsync(d_scored.as<void>(), 0, sizeof(unsigned long long), job->stream));
-    SQSG_HIP(hipMemsetAsync(d_next.as<void>(), 0, sizeof(unsigned int), job->stream));
+    SQSG_HIP(hipMemsetAsync(ctl + 8, 0, 24, job->stream));  // record, scored, segment counts
     SQSG_HIP(hipEventRecord(job->ev0, job->stream));
     SQSG_HIP(launch_variant(p, job, variant, blocks, W, job->stream));
     SQSG_HIP(hipEventRecord(job->ev1, job->stream));
     unsigned int count = 0;
     unsigned long long scored = 0;
-    SQSG_HIP(hipMemcpyAsync(&count, d_count.as<void>(), sizeof(count), hipMemcpyDeviceToHost,
+    SQSG_HIP(hipMemcpyAsync(&count, p.rec_count, sizeof(count), hipMemcpyDeviceToHost,
                             job->stream));
-    SQSG_HIP(hipMemcpyAsync(&scored, d_scored.as<void>(), sizeof(scored), hipMemcpyDeviceToHost,
+    SQSG_HIP(hipMemcpyAsync(&scored, p.scored, sizeof(scored), hipMemcpyDeviceToHost,
                             job->stream));
     SQSG_HIP(hipStreamSynchronize(job->stream));
     float ms = 0;
     SQSG_HIP(hipEventElapsedTime(&ms, job->ev0, job->ev1));
     job->kernel_ms += ms;
     job->launches++;
-    if (count > sink.cap) {  // sink overflow: widen and run the same range again
-      sink.ensure(count + count / 4, stride, p.nprob);
+    if (count > sink.cap) {  // sink overflow: run the same range again, wider or shorter
+      if (!fixed) {
+        sink.ensure(count + count / 4, stride, p.nprob);
+      } else if (end - begin > 1) {
+        limit = (end - begin) / 2;
+      } else {
+        throw std::out_of_range("the records of one candidate exceed the scratch buffer");
+      }
       continue;
     }
     if (scored != end - begin) throw DeviceError("device scored an unexpected candidate count");
@@ -842,11 +883,12 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
     const unsigned long long hk = search_key(job);
     if (hk != thr) {
       thr = hk;
-      SQSG_HIP(hipMemcpy(d_thr.as<void>(), &thr, sizeof(thr), hipMemcpyHostToDevice));
+      SQSG_HIP(hipMemcpy(ctl, &thr, sizeof(thr), hipMemcpyHostToDevice));
     }
     job->stats.finished = job->scored;
     begin = end;
     ++ramp;
+    if (limit < launch) limit = std::min(launch, 2 * limit);
     if (cb) {
       const auto now = clock::now();
       job->stats.timing_loop_ns =
@@ -854,7 +896,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user) {
       if (cb(&job->stats, user)) job->cancel.store(true);
     }
   }
-  finalize_results(job);
+  finalize_results(job, fixed == nullptr);
   const auto t_end = clock::now();
   job->stats.timing_loop_ns =
       std::chrono::duration_cast<std::chrono::nanoseconds>(t_end - t_loop).count();

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <map>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -181,22 +182,39 @@ struct sqsg_job {
 
 namespace sqsg {
 
+// Device memory a run uses instead of allocating its own: control words and the record sink.
+struct FixedScratch {
+  void *base = nullptr;
+  size_t bytes = 0;
+};
 sqsg_job *create_job(const sqsg_config &cfg);
-void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user);
+// a run of the job; with `fixed`, its threshold, counters and record sink live in that
+// memory and the run allocates no device memory (the job's inputs must be resident)
+void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user,
+             const FixedScratch *fixed = nullptr);
+// runs fn, translating exceptions into an SQSG_* status and the thread's last error text
+int guarded(const std::function<void()> &fn);
 void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint32_t B,
                  int32_t *d_bonds, double *d_obj, double *d_sro, int kernel, hipStream_t s);
 // helpers shared with the batched and serving drivers
 RunParams make_run_params(sqsg_job *job, int variant, uint32_t waves_per_block,
                           uint32_t *blocks);
 // Device record sink of one job: candidates at or below the threshold with their
-// objectives and packed species.
+// objectives and packed species.  Its arrays are its own allocations, or memory the caller
+// lends it (place).
 struct RecordSink {
-  DevBuf cand, obj, sub, species;
+  DevBuf cand_buf, obj_buf, sub_buf, species_buf;
+  uint64_t *cand = nullptr;
+  double *obj = nullptr, *sub = nullptr;
+  uint8_t *species = nullptr;
   uint32_t cap = 0;
   void ensure(uint32_t n, uint32_t stride, uint32_t nprob);
+  // carves a sink out of `bytes` of device memory at `base` (256-byte aligned); returns
+  // its capacity in records
+  uint32_t place(void *base, size_t bytes, uint32_t stride, uint32_t nprob);
   void bind(RunParams &p) const;
-  // copies `count` records of the candidate range [begin, end) back and merges them into
-  // the job's collection in candidate order
+  // copies `count` records of the candidate ranges back and merges them into the job's
+  // collection in candidate order
   void drain(sqsg_job *job, uint32_t count, uint32_t nprob,
              const std::vector<std::pair<uint64_t, uint64_t>> &ranges, hipStream_t s);
 };
@@ -217,7 +235,9 @@ unsigned long long merge_records(sqsg_job *job, HostRecords &&recs);
 unsigned long long search_key(const sqsg_job *job);
 // block of the reference's thread-pool split that candidate t belongs to
 uint64_t block_of(const sqsg_job *job, uint64_t t);
-void finalize_results(sqsg_job *job);
+// flattens the collection into the result set; eager_sro re-scores the SRO cubes on the
+// device now (else on first request)
+void finalize_results(sqsg_job *job, bool eager_sro = true);
 // accessors of finalized result r (original site order, atomic numbers)
 void result_species_z(const sqsg_job *job, uint64_t r, uint8_t *out);
 std::string result_rank(const sqsg_job *job, uint64_t r);

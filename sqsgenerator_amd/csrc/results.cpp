@@ -158,7 +158,7 @@ unsigned long long merge_records(sqsg_job *job, HostRecords &&recs) {
   return search_key(job);
 }
 
-void finalize_results(sqsg_job *job) {
+void finalize_results(sqsg_job *job, bool eager_sro) {
   ResultSet &R = job->res;
   R.clear();
   if (!job->coll.pending.empty())
@@ -175,7 +175,7 @@ void finalize_results(sqsg_job *job) {
   job->coll.clear();
   job->coll.search = DBL_MAX;
   // SRO cubes are re-scored on the device: eagerly for packs of ordinary size
-  if (R.size() && R.size() <= (uint64_t(1) << 20)) {
+  if (eager_sro && R.size() && R.size() <= (uint64_t(1) << 20)) {
     std::vector<double> sro(R.size() * job->cube_total);
     result_sro(job, 0, R.size(), sro.data());
     R.sro = std::move(sro);
