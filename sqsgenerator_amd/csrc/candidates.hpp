@@ -111,34 +111,42 @@ __device__ __forceinline__ bool block_start(const RP &p, uint64_t cand) {
                             : (cand - p.blk_split) % p.blk_size == 0;
 }
 
-// Record bound of the wavefront filter as it stands (before the current candidates are
-// offered, so it holds only candidates that precede them): the order key of its L-th level,
-// or no bound.
+// Bound of the wavefront filter as it stands before a round (it holds only candidates that
+// precede the round's): the order key of its L-th level, or no bound.  The early exit uses it.
 template <class RP>
 __device__ __forceinline__ unsigned long long filter_bound(const LevelFilter &lf,
                                                            const RP &p) {
   return p.wave_filter && lf.n == p.L ? okey(lf.lvl[p.L - 1]) : ~0ull;
 }
 
-// Offers the objectives of the lanes in `pass` (a ballot) to the filter.
+// Offers the objectives of the lanes in `mask` (a ballot; lanes hold ascending candidates)
+// to the filter one lane at a time and returns the lanes that were at or below the filter's
+// L-th level when their turn came (or met a filter of fewer than L levels): the others
+// have L smaller distinct objectives among the candidates before them.  Without the filter,
+// every lane of `mask` passes.
 template <class RP>
-__device__ __forceinline__ void offer_levels(LevelFilter &lf, const RP &p,
-                                             unsigned long long pass, double tot) {
-  if (!p.wave_filter) return;
-  while (pass) {
-    const int l = __ffsll(pass) - 1;
-    pass &= pass - 1;
+__device__ __forceinline__ unsigned long long filter_pass(LevelFilter &lf, const RP &p,
+                                                          unsigned long long mask,
+                                                          double tot) {
+  if (!p.wave_filter) return mask;
+  unsigned long long pass = 0;
+  while (mask) {
+    const int l = __ffsll(mask) - 1;
+    mask &= mask - 1;
     const double o = __shfl(tot, l);
+    const bool full = lf.n == p.L;
+    if (!full || o <= lf.lvl[p.L - 1]) pass |= 1ull << l;
     bool present = false;
     for (uint32_t i = 0; i < lf.n; ++i) present |= (lf.lvl[i] == o);
-    if (present || !(lf.n < p.L || o < lf.lvl[lf.n - 1])) continue;
-    uint32_t pos = lf.n < p.L ? lf.n++ : lf.n - 1;
+    if (present || (full && !(o < lf.lvl[lf.n - 1]))) continue;
+    uint32_t pos = full ? lf.n - 1 : lf.n++;
     while (pos > 0 && lf.lvl[pos - 1] > o) {
       lf.lvl[pos] = lf.lvl[pos - 1];
       --pos;
     }
     lf.lvl[pos] = o;
   }
+  return pass;
 }
 
 // Next chunk of a persistent launch's work list for this wavefront: walks the items in

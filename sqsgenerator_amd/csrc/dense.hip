@@ -407,20 +407,25 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
     for (uint32_t q = 0; q < npass; ++q)
       if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
     wave_sync();
-    // the job threshold and the wavefront filter's bound before this round: both hold for
-    // every candidate of the round, so they serve the early exit and the record test
-    const unsigned long long g =
-        __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long lk = filter_bound(lf, p);
-    const unsigned long long prune_key = p.prune ? (g < lk ? g : lk) : ~0ull;
+    // early exit bound: the job threshold and the wavefront filter's bound (neither changes
+    // during the round, so the record test below reads the same values after scoring)
+    unsigned long long prune_key = ~0ull;
+    if (p.prune) {
+      const unsigned long long g0 =
+          __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long l0 = filter_bound(lf, p);
+      prune_key = g0 < l0 ? g0 : l0;
+    }
     const double tot =
         dense_score<TT, G, D, kDenseTermTables<TT>>(pr, rows, stride, cnt, CW, npass, live,
                                                     lane, nullptr, nullptr, 0u, am, prune_key);
     scored += static_cast<unsigned long long>(__popcll(am));
-    // a candidate above the threshold or above the filter's L-th level cannot be inserted
+    // a candidate above the threshold, or above the filter's L-th level when its turn
+    // comes, cannot be inserted
+    const unsigned long long g =
+        __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const bool below = active && okey(tot) <= g;
-    const bool pass = below && okey(tot) <= lk;
-    offer_levels(lf, p, __ballot(below), tot);
+    const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
     if (pass) {
       const uint32_t slot = atomicAdd(p.rec_count, 1u);
       if (slot < p.rec_cap) {
@@ -711,10 +716,8 @@ __device__ __forceinline__ void ws_record(const RP &p, const uint8_t *rows, uint
   if (lane == 0) atomicAdd(p.scored, static_cast<unsigned long long>(__popcll(am)));
   const unsigned long long g =
       __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned long long lk = filter_bound(lf, p);
   const bool below = active && okey(tot) <= g;
-  const bool pass = below && okey(tot) <= lk;
-  offer_levels(lf, p, __ballot(below), tot);
+  const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
   if (pass) {
     const uint32_t slot = atomicAdd(p.rec_count, 1u);
     if (slot < p.rec_cap) {

@@ -742,31 +742,34 @@ void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob,
                        const std::vector<std::pair<uint64_t, uint64_t>> &ranges,
                        hipStream_t s) {
   const uint32_t stride = job->stride;
-  std::vector<uint64_t> c(count);
-  std::vector<double> o(count), sb(size_t(count) * nprob);
-  std::vector<uint8_t> sp(size_t(count) * stride);
+  auto blk = std::make_shared<RecordBlock>();
+  blk->cand.resize(count);
+  blk->obj.resize(count);
+  blk->sub.resize(size_t(count) * nprob);
+  blk->species.resize(size_t(count) * stride);
   if (count) {
-    SQSG_HIP(hipMemcpyAsync(c.data(), cand, count * sizeof(uint64_t), hipMemcpyDeviceToHost,
+    SQSG_HIP(hipMemcpyAsync(blk->cand.data(), cand, count * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipMemcpyAsync(blk->obj.data(), obj, count * sizeof(double), hipMemcpyDeviceToHost,
                             s));
-    SQSG_HIP(hipMemcpyAsync(o.data(), obj, count * sizeof(double), hipMemcpyDeviceToHost, s));
-    SQSG_HIP(hipMemcpyAsync(sb.data(), sub, sb.size() * sizeof(double), hipMemcpyDeviceToHost,
-                            s));
-    SQSG_HIP(hipMemcpyAsync(sp.data(), species, sp.size(), hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipMemcpyAsync(blk->sub.data(), sub, blk->sub.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    SQSG_HIP(hipMemcpyAsync(blk->species.data(), species, blk->species.size(),
+                            hipMemcpyDeviceToHost, s));
     SQSG_HIP(hipStreamSynchronize(s));
   }
-  // one record set per candidate range (a step can hold a re-issued range and the next one)
+  // the threshold only decreases as candidates are merged, so a record above it now is
+  // above it at its turn: it is dropped before the sort
+  const double search = job->coll.search;
   for (const auto &[begin, end] : ranges) {
     HostRecords r;
     r.begin = begin;
     r.end = end;
-    for (uint32_t k = 0; k < count; ++k) {
-      if (c[k] < begin || c[k] >= end) continue;
-      r.cand.push_back(c[k]);
-      r.obj.push_back(o[k]);
-      r.sub.insert(r.sub.end(), sb.begin() + size_t(k) * nprob, sb.begin() + size_t(k + 1) * nprob);
-      r.species.insert(r.species.end(), sp.begin() + size_t(k) * stride,
-                       sp.begin() + size_t(k + 1) * stride);
-    }
+    r.block = blk;
+    for (uint32_t k = 0; k < count; ++k)
+      if (blk->cand[k] >= begin && blk->cand[k] < end && blk->obj[k] <= search)
+        r.order.emplace_back(blk->cand[k], k);
+    std::sort(r.order.begin(), r.order.end());
     merge_records(job, std::move(r));
   }
 }

@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <map>
+#include <memory>
 #include <cstdint>
 #include <functional>
 #include <stdexcept>
@@ -109,12 +110,19 @@ struct LevelRecords {
   size_t size() const { return cand.size(); }
 };
 
-// Device records of one candidate range [begin, end), copied back to the host.
-struct HostRecords {
-  uint64_t begin = 0, end = 0;
+// One copy of a record sink's contents on the host.
+struct RecordBlock {
   std::vector<uint64_t> cand;
   std::vector<double> obj, sub;
   std::vector<uint8_t> species;
+};
+
+// The device records of one candidate range [begin, end): (candidate, record) pairs of a
+// block, in candidate order.
+struct HostRecords {
+  uint64_t begin = 0, end = 0;
+  std::shared_ptr<const RecordBlock> block;
+  std::vector<std::pair<uint64_t, uint32_t>> order;
 };
 
 // The reference's result collection and search threshold, applied to the job's candidate

@@ -171,12 +171,10 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
     scored += static_cast<unsigned long long>(__popcll(__ballot(active)));
     const unsigned long long g =
         __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // a candidate above the threshold or above the filter's L-th level (as it stood before
-    // this round) cannot be inserted
-    const unsigned long long lk = filter_bound(lf, p);
+    // a candidate above the threshold, or above the filter's L-th level when its turn
+    // comes, cannot be inserted
     const bool below = active && okey(tot) <= g;
-    const bool pass = below && okey(tot) <= lk;
-    offer_levels(lf, p, __ballot(below), tot);
+    const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
     if (pass) {
       const uint32_t slot = atomicAdd(p.rec_count, 1u);
       if (slot < p.rec_cap) {

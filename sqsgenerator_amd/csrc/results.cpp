@@ -96,14 +96,9 @@ void merge_in_order(sqsg_job *job, const HostRecords &r) {
   const uint32_t stride = job->stride;
   const bool systematic = job->iteration_mode == SQSG_ITERATION_SYSTEMATIC;
   const bool stalls = job->sublattice_mode == SQSG_SUBLATTICE_INTERACT;
-  const size_t n = r.cand.size();
-  std::vector<uint32_t> order(n);
-  for (size_t k = 0; k < n; ++k) order[k] = static_cast<uint32_t>(k);
-  std::sort(order.begin(), order.end(),
-            [&](uint32_t a, uint32_t b) { return r.cand[a] < r.cand[b]; });
-  for (uint32_t k : order) {
-    const uint64_t t = r.cand[k];
-    const double o = r.obj[k];
+  const RecordBlock &B = *r.block;
+  for (const auto &[t, k] : r.order) {
+    const double o = B.obj[k];
     if (!(o <= C.search)) continue;
     if (job->max_results >= 0 && C.stalled_block != ~uint64_t(0) &&
         block_of(job, t) == C.stalled_block)
@@ -114,7 +109,7 @@ void merge_in_order(sqsg_job *job, const HostRecords &r) {
       if (stalls) C.stalled_block = block_of(job, t);
       continue;
     }
-    const uint8_t *sp = r.species.data() + size_t(k) * stride;
+    const uint8_t *sp = B.species.data() + size_t(k) * stride;
     const bool fresh = it == C.levels.end();
     if (fresh) it = C.levels.emplace(o, LevelRecords{}).first;
     LevelRecords &lv = it->second;
@@ -127,8 +122,8 @@ void merge_in_order(sqsg_job *job, const HostRecords &r) {
       lv.seen.emplace(h, static_cast<uint32_t>(lv.size()));
     }
     lv.cand.push_back(t);
-    lv.sub.insert(lv.sub.end(), r.sub.begin() + size_t(k) * nprob,
-                  r.sub.begin() + size_t(k + 1) * nprob);
+    lv.sub.insert(lv.sub.end(), B.sub.begin() + size_t(k) * nprob,
+                  B.sub.begin() + size_t(k + 1) * nprob);
     lv.species.insert(lv.species.end(), sp, sp + stride);
     if (o < job->stats.best_objective) {
       job->stats.best_objective = o;
