@@ -249,17 +249,21 @@ SQSG_API int sqsg_job_kernel_shape(sqsg_job *job, char *buf, size_t len);
 
 /* ---- serving: continuous batching with device-resident job state --------------------- */
 typedef struct sqsg_server_options {
-  uint64_t hbm_budget_bytes;     /* admission limit on resident job footprint */
+  uint64_t hbm_budget_bytes;     /* admission limit on resident job footprint (0: 80% of the
+                                    device memory free when the server is created) */
   uint32_t max_active;           /* admission limit on resident jobs */
   uint64_t slice_candidates;     /* per-job candidates per scheduling step */
 } sqsg_server_options;
 SQSG_API int sqsg_server_create(const sqsg_server_options *opt, sqsg_server **out);
-/* Queues a created job; returns a ticket.  The server owns nothing: the caller keeps the
- * job and must not touch it until sqsg_server_poll reports it done. */
+/* Queues a created job; returns a ticket.  A host thread of the server uploads the job's
+ * inputs while steps run.  The server owns nothing: the caller keeps the job and must not
+ * touch it until sqsg_server_poll reports it done. */
 SQSG_API int sqsg_server_submit(sqsg_server *srv, sqsg_job *job, uint64_t *ticket);
-/* One scheduling step: admits queued jobs while the footprint and resident-job limits
- * allow, then advances every resident job by one persistent launch and retires finished
- * ones; reports the resident and queued job counts after the step. */
+/* One scheduling step: admits queued jobs whose inputs are uploaded, in submission order,
+ * skipping those that do not fit the memory left, while the resident-job limit allows; then
+ * advances every resident job by one persistent step and retires finished ones; reports the
+ * resident and queued job counts after the step.  A job whose preparation failed is marked
+ * done and its error returned here. */
 SQSG_API int sqsg_server_step(sqsg_server *srv, int *active_jobs, int *queued_jobs);
 /* Status of a ticket: done flag and submission, admission and completion times in seconds
  * since server creation (-1 where not reached yet). */

@@ -777,7 +777,7 @@ class Server:
     """Continuous batching: submitted jobs are admitted by device-memory footprint between
     persistent steps and advance together with the jobs already resident."""
 
-    def __init__(self, hbm_budget_bytes: int = 8 << 30, max_active: int = 64,
+    def __init__(self, hbm_budget_bytes: int = 0, max_active: int = 64,
                  slice_candidates: int = 0):
         opt = L.ServerOptions(hbm_budget_bytes, max_active, slice_candidates)
         h = C.c_void_p()

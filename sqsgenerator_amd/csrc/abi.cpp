@@ -240,8 +240,8 @@ int sqsg_score_batch(sqsg_job *job, uint32_t p, const uint8_t *species, uint32_t
     sqsg::DevBuf d_sro(sro ? size_t(B) * nb * sizeof(double) : 0);
     SQSG_HIP(hipMemcpy(d_sp.as<void>(), species, size_t(B) * P.N, hipMemcpyHostToDevice));
     sqsg::score_batch(job, p, d_sp.as<uint8_t>(), B, bonds ? d_bonds.as<int32_t>() : nullptr,
-                      d_obj.as<double>(), sro ? d_sro.as<double>() : nullptr, kernel, job->stream);
-    SQSG_HIP(hipStreamSynchronize(job->stream));
+                      d_obj.as<double>(), sro ? d_sro.as<double>() : nullptr, kernel, sqsg::job_stream(job));
+    SQSG_HIP(hipStreamSynchronize(sqsg::job_stream(job)));
     SQSG_HIP(hipMemcpy(objective, d_obj.as<void>(), size_t(B) * sizeof(double),
                        hipMemcpyDeviceToHost));
     if (bonds)

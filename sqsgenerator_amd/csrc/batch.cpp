@@ -12,6 +12,7 @@
 // merges its records, tightens each job's threshold and admits or retires jobs (which is
 // what the serving loop builds on); the device never waits for the host between steps.
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -19,6 +20,10 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <condition_variable>
+#include <exception>
+#include <thread>
+#include <atomic>
 
 #include "job.hpp"
 #include "kernels.hpp"
@@ -27,42 +32,55 @@ namespace sqsg {
 
 namespace {
 
+// held around graph captures and around device uploads made on other host threads
+std::mutex &capture_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 constexpr uint32_t kGroupJobs = 256;          // resident jobs per group
 constexpr uint32_t kGroupItems = 2 * kGroupJobs;  // a job's re-issued range + its next slice
 
 struct Group;
 
-// Host copies of a slot's device counters, in pinned memory so the copies are asynchronous.
-struct HostCounters {
-  unsigned long long scored[2];
-  unsigned long long thr;
-  unsigned int count[2];
-  unsigned int pad[2];
-};
-
 using Range = std::pair<uint64_t, uint64_t>;
 
 // Device buffers of a resident job, pooled by the engine across jobs: record sinks and
 // counters double-buffered per step parity (a step writes one set while the host reads the
-// other), the threshold, and pinned host copies of the counters.
+// other), and the stages in pinned host memory that the stage kernel fills at the end of
+// each step (counts and the first records).
 struct SlotBuffers {
   RecordSink sink[2];
   DevBuf count[2], scored[2];
-  DevBuf thr;
-  HostCounters *hc = nullptr;
+  uint8_t *stage[2] = {nullptr, nullptr};      // host addresses
+  uint8_t *stage_dev[2] = {nullptr, nullptr};  // the same memory as the device sees it
+  size_t stage_cap = 0;
   SlotBuffers() {
     for (int q = 0; q < 2; ++q) {
       count[q].alloc(sizeof(unsigned int));
       scored[q].alloc(sizeof(unsigned long long));
     }
-    thr.alloc(sizeof(unsigned long long));
-    void *h = nullptr;
-    SQSG_HIP(hipHostMalloc(&h, sizeof(HostCounters), hipHostMallocDefault));
-    hc = static_cast<HostCounters *>(h);
   }
-  ~SlotBuffers() {
-    if (hc) (void)hipHostFree(hc);
+  void ensure_stage(size_t bytes) {
+    if (bytes <= stage_cap) return;
+    release_stage();
+    for (int q = 0; q < 2; ++q) {
+      void *h = nullptr, *d = nullptr;
+      SQSG_HIP(hipHostMalloc(&h, bytes, hipHostMallocMapped | hipHostMallocCoherent));
+      stage[q] = static_cast<uint8_t *>(h);
+      SQSG_HIP(hipHostGetDevicePointer(&d, h, 0));
+      stage_dev[q] = static_cast<uint8_t *>(d);
+    }
+    stage_cap = bytes;
   }
+  void release_stage() {
+    for (int q = 0; q < 2; ++q) {
+      if (stage[q]) (void)hipHostFree(stage[q]);
+      stage[q] = stage_dev[q] = nullptr;
+    }
+    stage_cap = 0;
+  }
+  ~SlotBuffers() { release_stage(); }
 };
 
 struct Slot {
@@ -72,12 +90,12 @@ struct Slot {
   std::unique_ptr<SlotBuffers> buf;
   RecordSink *sink = nullptr;   // buf->sink
   DevBuf *count = nullptr, *scored = nullptr;
-  HostCounters *hc = nullptr;
   uint64_t next = 0;          // first candidate not issued yet
   uint64_t slice = 0;         // candidates per step
   uint32_t ramp = 0;          // steps issued so far
   bool ramp_up = false;       // early steps are shorter (records not bounded by the filter)
   uint64_t chunk = 0;         // candidates a wavefront claims at once (one segment)
+  double cost = 0;            // relative device work per candidate
   std::vector<Range> issued[2];  // ranges of the step of each parity
   std::deque<Range> retry;    // ranges to issue again (their record sink overflowed)
   size_t footprint = 0;       // device bytes held
@@ -98,6 +116,7 @@ struct Group {
   WorkItem *h_items[2] = {nullptr, nullptr};
   uint32_t *h_counts[2] = {nullptr, nullptr};
   uint32_t graph_wave_lds = 0, graph_blocks = 0;
+  double work = 0;  // estimated cost of the current step (candidates x pair work)
   std::vector<Slot *> members;
   Group() {
     for (int q = 0; q < 2; ++q) {
@@ -134,10 +153,12 @@ class BatchEngine {
  public:
   explicit BatchEngine(bool use_graph) : use_graph_(use_graph) {
     SQSG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SQSG_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
     SQSG_HIP(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking));
     for (int q = 0; q < 2; ++q) {
       SQSG_HIP(hipEventCreate(&ev_start_[q]));
       SQSG_HIP(hipEventCreate(&ev_done_[q]));
+      SQSG_HIP(hipEventCreateWithFlags(&ev_join_[q], hipEventDisableTiming));
     }
   }
   ~BatchEngine() {
@@ -150,8 +171,10 @@ class BatchEngine {
     for (int q = 0; q < 2; ++q) {
       if (ev_start_[q]) (void)hipEventDestroy(ev_start_[q]);
       if (ev_done_[q]) (void)hipEventDestroy(ev_done_[q]);
+      if (ev_join_[q]) (void)hipEventDestroy(ev_join_[q]);
     }
     if (copy_) (void)hipStreamDestroy(copy_);
+    if (stream2_) (void)hipStreamDestroy(stream2_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -181,6 +204,12 @@ class BatchEngine {
     }
     // a paired block claims one batch for its four pairs at a time
     s->chunk = gen * s->p.rounds * (variant == SQSG_KERNEL_DENSE_PAIRED ? kDensePairs : 1);
+    if (dense) {  // every grid position of the tile bound costs its MFMAs in every shell
+      const double TT = dense_tile_bound_of(job);
+      s->cost = job->problems[0].S * TT * (TT + 1) / 2 / s->p.dense_cw;
+    } else {
+      s->cost = static_cast<double>(work) / 64;
+    }
     s->slice = std::max(s->slice, s->chunk);
     s->p.rec_stride = job->stride;
     if (!pool_.empty()) {
@@ -192,16 +221,14 @@ class BatchEngine {
     s->sink = s->buf->sink;
     s->count = s->buf->count;
     s->scored = s->buf->scored;
-    s->hc = s->buf->hc;
     const uint32_t cap =
         first_sink_records(job, uint64_t(device_cu_count()) * 8, s->slice);
     s->ramp_up = !records_bounded(job);
     for (int q = 0; q < 2; ++q) s->sink[q].ensure(cap, job->stride, s->p.nprob);
-    *s->hc = HostCounters{};
-    s->hc->thr = search_key(job);
-    SQSG_HIP(hipMemcpyAsync(s->buf->thr.as<void>(), &s->hc->thr, sizeof(s->hc->thr),
-                            hipMemcpyHostToDevice, stream_));
-    s->p.thr_key = s->buf->thr.as<unsigned long long>();
+    // the stage holds a probe's records (the first step of a served job) as well
+    s->p.stage_cap = std::min<uint32_t>(cap, 4096);
+    s->buf->ensure_stage(stage_bytes(s->p.stage_cap, s->p.nprob, job->stride));
+    s->p.thr_key = search_key(job);  // travels in the job table of every step
     s->footprint = footprint(*s);
     // groups share a launch shape: the tile bound (dense), also K (paired: one LDS
     // layout), the largest species count (lane)
@@ -242,8 +269,20 @@ class BatchEngine {
       std::vector<Group *> live;
       for (auto &[k, g] : groups_)
         if (!g->members.empty()) live.push_back(prepare_group(*g, p));
+      // the groups' persistent kernels run on two streams, so that one group's kernel fills
+      // the tail of another's; heaviest group first, each to the stream with less work
+      std::sort(live.begin(), live.end(),
+                [](const Group *a, const Group *b) { return a->work > b->work; });
+      double load[2] = {0, 0};
       SQSG_HIP(hipEventRecord(ev_start_[p], stream_));
-      for (Group *g : live) launch_group(*g);
+      SQSG_HIP(hipStreamWaitEvent(stream2_, ev_start_[p], 0));
+      for (Group *g : live) {
+        const int k = load[1] < load[0] ? 1 : 0;
+        load[k] += g->work;
+        launch_group(*g, k ? stream2_ : stream_);
+      }
+      SQSG_HIP(hipEventRecord(ev_join_[p], stream2_));
+      SQSG_HIP(hipStreamWaitEvent(stream_, ev_join_[p], 0));
       SQSG_HIP(hipEventRecord(ev_done_[p], stream_));
       in_flight_[p] = true;
       host_ns_ += std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -258,6 +297,7 @@ class BatchEngine {
   double kernel_ms = 0;
   uint64_t candidates = 0, launches = 0;
   uint32_t last_items = 0;
+  bool probe_new_ = false;  // new jobs start with a probe range (serving)
 
  private:
   // returns the buffers of retired slots to the pool
@@ -279,20 +319,26 @@ class BatchEngine {
     SQSG_HIP(hipEventElapsedTime(&ms, ev_start_[q], ev_done_[q]));
     kernel_ms += ms;
     ++launches;
-    for (auto &s : slots_) {
-      if (s->issued[q].empty()) continue;
-      SQSG_HIP(hipMemcpyAsync(&s->hc->count[q], s->count[q].as<void>(), sizeof(unsigned int),
-                              hipMemcpyDeviceToHost, copy_));
-      SQSG_HIP(hipMemcpyAsync(&s->hc->scored[q], s->scored[q].as<void>(),
-                              sizeof(unsigned long long), hipMemcpyDeviceToHost, copy_));
-    }
-    SQSG_HIP(hipStreamSynchronize(copy_));
+    std::atomic_thread_fence(std::memory_order_acquire);  // the stages were written
+    // records are copied to the host here, then merged (jobs in parallel: each merge
+    // touches only its own job's collection)
+    struct MergeTask {
+      Slot *slot;
+      std::shared_ptr<const RecordBlock> blk;
+      std::vector<Range> ranges;
+    };
+    std::vector<MergeTask> tasks;
+    size_t total_records = 0;
     for (auto &s : slots_) {
       sqsg_job *job = s->job;
       if (!s->issued[q].empty()) {
         uint64_t expect = 0;
         for (const Range &r : s->issued[q]) expect += r.second - r.first;
-        const unsigned int count = s->hc->count[q];
+        const uint8_t *st = s->buf->stage[q];
+        unsigned int count = 0;
+        unsigned long long scored = 0;
+        std::memcpy(&count, st, sizeof(count));
+        std::memcpy(&scored, st + 8, sizeof(scored));
         job->launches++;
         job->kernel_ms += ms;  // shared launch: every resident job saw the whole step
         if (count > s->sink[q].cap) {
@@ -301,19 +347,17 @@ class BatchEngine {
           s->footprint = footprint(*s);
           for (const Range &r : s->issued[q]) s->retry.push_back(r);
         } else {
-          if (s->hc->scored[q] != expect)
+          if (scored != expect)
             throw DeviceError("persistent launch scored an unexpected candidate count");
           job->scored += expect;
           candidates += expect;
-          s->sink[q].drain(job, count, s->p.nprob, s->issued[q], copy_);
-          // the host's threshold after the candidates merged so far, ordered before the
-          // next step (the step running now keeps the older, looser one)
-          const unsigned long long hk = search_key(job);
-          if (hk < s->hc->thr) {
-            s->hc->thr = hk;
-            SQSG_HIP(hipMemcpyAsync(s->buf->thr.as<void>(), &s->hc->thr, sizeof(s->hc->thr),
-                                    hipMemcpyHostToDevice, stream_));
-          }
+          // many records (a job's first steps) come from the device sink, not the stage
+          tasks.push_back({s.get(),
+                           count <= s->p.stage_cap
+                               ? stage_block(job, st, s->p.stage_cap, count, s->p.nprob)
+                               : s->sink[q].fetch(job, count, s->p.nprob, copy_),
+                           s->issued[q]});
+          total_records += count;
           job->stats.finished = job->scored;
         }
         s->issued[q].clear();
@@ -321,6 +365,32 @@ class BatchEngine {
       const bool exhausted = s->next >= job->iterations || job->cancel.load();
       if (exhausted && s->retry.empty() && s->issued[q ^ 1].empty()) s->done = true;
     }
+    auto merge = [&tasks](size_t k) { merge_block(tasks[k].slot->job, tasks[k].blk, tasks[k].ranges); };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (total_records < 16384 || tasks.size() < 2 || hw == 1) {
+      for (size_t k = 0; k < tasks.size(); ++k) merge(k);
+    } else {
+      std::atomic<size_t> next{0};
+      std::vector<std::thread> pool;
+      std::exception_ptr err;
+      std::mutex err_mutex;
+      for (unsigned t = 0; t < std::min<size_t>(hw, tasks.size()); ++t)
+        pool.emplace_back([&] {
+          for (size_t k; (k = next.fetch_add(1)) < tasks.size();) {
+            try {
+              merge(k);
+            } catch (...) {
+              std::lock_guard<std::mutex> lock(err_mutex);
+              err = std::current_exception();
+            }
+          }
+        });
+      for (auto &t : pool) t.join();
+      if (err) std::rethrow_exception(err);
+    }
+    // the host's threshold after the candidates merged so far goes into the job table of
+    // the next step (the step running now keeps the older, looser one)
+    for (const MergeTask &t : tasks) t.slot->p.thr_key = search_key(t.slot->job);
     const auto t1 = std::chrono::steady_clock::now();
     const int64_t ns =
         std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count() + host_ns_;
@@ -332,7 +402,9 @@ class BatchEngine {
         continue;
       }
       sqsg_job *job = (*it)->job;
-      finalize_results(job);
+      // SRO cubes are re-scored when first read: device allocations here would wait for the
+      // step that is running
+      finalize_results(job, false);
       job->stats.timing_total_ns = job->stats.timing_loop_ns;
       Group &g = *(*it)->group;
       g.members.erase(std::find(g.members.begin(), g.members.end(), it->get()));
@@ -364,12 +436,14 @@ class BatchEngine {
     RunParams *table = g.h_table[q];
     WorkItem *items = g.h_items[q];
     uint32_t n = 0;
+    g.work = 0;
     uint32_t wave_lds = 0;
     for (uint32_t k = 0; k < g.members.size(); ++k) {
       Slot *s = g.members[k];
       s->sink[q].bind(s->p);
       s->p.rec_count = s->count[q].as<unsigned int>();
       s->p.scored = s->scored[q].as<unsigned long long>();
+      s->p.stage = s->buf->stage_dev[q];
       table[k] = s->p;
       wave_lds = std::max(wave_lds, s->p.wave_lds);
       s->issued[q].clear();
@@ -377,21 +451,26 @@ class BatchEngine {
         s->issued[q].push_back(s->retry.front());
         s->retry.pop_front();
       }
-      if (s->next < s->job->iterations && !s->job->cancel.load()) {
+      // a job admitted to a running engine first scores a probe of 4096 candidates and
+      // takes its next range once the probe is merged, so that its threshold is set: with
+      // the threshold still unset, every wavefront records its first candidates of the job
+      const bool waiting = probe_new_ && s->ramp == 1 && s->job->coll.merged < s->next;
+      if (s->next < s->job->iterations && !s->job->cancel.load() && !waiting) {
         // without the wavefront filter the device records every candidate at or below the
         // threshold as it stood before the step: such steps grow from 4096 candidates by 4x
         // up to the slice
+        const uint64_t ramped = std::max(s->chunk, uint64_t(4096) << std::min(2 * s->ramp, 40u));
         const uint64_t size =
-            s->ramp_up ? std::min(s->slice, std::max(s->chunk, uint64_t(4096)
-                                                                   << std::min(2 * s->ramp, 40u)))
-                       : s->slice;
+            std::min(s->slice, s->ramp_up || (probe_new_ && s->ramp == 0) ? ramped : s->slice);
         ++s->ramp;
         const uint64_t end = std::min(s->job->iterations, s->next + size);
         s->issued[q].push_back(Range{s->next, end});
         s->next = end;
       }
-      for (const Range &r : s->issued[q])
+      for (const Range &r : s->issued[q]) {
         items[n++] = WorkItem{k, static_cast<uint32_t>(s->chunk), r.first, r.second};
+        g.work += double(r.second - r.first) * s->cost;
+      }
     }
     last_items = n;
     SQSG_HIP(hipMemcpyAsync(g.d_jobs.as<void>(), table, g.members.size() * sizeof(RunParams),
@@ -403,6 +482,9 @@ class BatchEngine {
     g.h_counts[q][1] = static_cast<uint32_t>(g.members.size());
     SQSG_HIP(hipMemcpyAsync(g.d_counts.as<void>(), g.h_counts[q], 2 * sizeof(uint32_t),
                             hipMemcpyHostToDevice, stream_));
+    // a captured graph keeps its LDS size while members come and go, so the size only grows
+    // (each change is a new capture)
+    if (use_graph_ && g.exec) wave_lds = std::max(wave_lds, g.graph_wave_lds);
     const size_t block_lds = wave_lds;
     uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / std::max<size_t>(block_lds, 1));
     per_cu = std::max(1u, std::min(per_cu, 16u));
@@ -429,12 +511,17 @@ class BatchEngine {
       if (g.exec) SQSG_HIP(hipGraphExecDestroy(g.exec));
       g.exec = nullptr;
       hipGraph_t graph = nullptr;
+      // no synchronous device call of another thread (a server's job preparation) may run
+      // while the stream is being captured
+      std::lock_guard<std::mutex> capture_lock(capture_mutex());
       SQSG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
       const hipError_t e1 = launch_batch_reset(bp, kGroupJobs, stream_);
-      const hipError_t e2 = launch_kernel(g);
+      const hipError_t e2 = launch_kernel(g, stream_);
+      const hipError_t e3s = launch_batch_stage(bp, kGroupJobs, stream_);
       SQSG_HIP(hipStreamEndCapture(stream_, &graph));
       SQSG_HIP(e1);
       SQSG_HIP(e2);
+      SQSG_HIP(e3s);
       const hipError_t e3 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       SQSG_HIP(e3);
@@ -444,29 +531,32 @@ class BatchEngine {
     return &g;
   }
 
-  hipError_t launch_kernel(const Group &g) {
+  hipError_t launch_kernel(const Group &g, hipStream_t st) {
     switch (g.variant) {
-      case SQSG_KERNEL_DENSE_MFMA: return launch_batch_dense(g.bp, g.shape, g.blocks, stream_);
+      case SQSG_KERNEL_DENSE_MFMA: return launch_batch_dense(g.bp, g.shape, g.blocks, st);
       case SQSG_KERNEL_DENSE_PAIRED:
-        return launch_batch_dense_paired(g.bp, g.shape, g.blocks, stream_);
-      default: return launch_batch_lane(g.bp, g.shape, g.blocks, 1, stream_);
+        return launch_batch_dense_paired(g.bp, g.shape, g.blocks, st);
+      default: return launch_batch_lane(g.bp, g.shape, g.blocks, 1, st);
     }
   }
 
   // the step's launches only (tables, attributes and graphs are prepared beforehand)
-  void launch_group(Group &g) {
+  void launch_group(Group &g, hipStream_t st) {
     if (!use_graph_) {
-      SQSG_HIP(launch_batch_reset(g.bp, kGroupJobs, stream_));
-      SQSG_HIP(launch_kernel(g));
+      SQSG_HIP(launch_batch_reset(g.bp, kGroupJobs, st));
+      SQSG_HIP(launch_kernel(g, st));
+      SQSG_HIP(launch_batch_stage(g.bp, kGroupJobs, st));
       return;
     }
-    SQSG_HIP(hipGraphLaunch(g.exec, stream_));
+    SQSG_HIP(hipGraphLaunch(g.exec, st));
   }
 
   bool use_graph_;
-  hipStream_t stream_ = nullptr;  // steps (and threshold updates)
-  hipStream_t copy_ = nullptr;    // read-back of finished steps
+  hipStream_t stream_ = nullptr;   // steps (table uploads, threshold updates, half the groups)
+  hipStream_t stream2_ = nullptr;  // the other half of a step's groups
+  hipStream_t copy_ = nullptr;     // read-back of finished steps
   hipEvent_t ev_start_[2] = {nullptr, nullptr}, ev_done_[2] = {nullptr, nullptr};
+  hipEvent_t ev_join_[2] = {nullptr, nullptr};
   bool in_flight_[2] = {false, false};
   int parity_ = 0;
   int64_t host_ns_ = 0;
@@ -547,11 +637,23 @@ void autotune(sqsg_job *job, int *chosen, double *rate) {
 
 // ---- serving ---------------------------------------------------------------------------
 
+// The kernel variant a job runs in the batched form; its device inputs (pair lists, the
+// dense tile stream) are uploaded here.
+void prepare_for_batch(sqsg_job *job) {
+  ensure_device(job);
+  const int variant = resolve_kernel(job, job->kernel == SQSG_KERNEL_SPARSE_WAVE
+                                              ? SQSG_KERNEL_AUTO : job->kernel);
+  if (variant == SQSG_KERNEL_DENSE_MFMA || variant == SQSG_KERNEL_DENSE_PAIRED)
+    ensure_dense(job);
+}
+
 struct Ticket {
-  sqsg_job *job;
+  sqsg_job *job = nullptr;
   double t_submit = -1, t_admit = -1, t_done = -1;
   bool done = false;
   size_t footprint = 0;
+  std::atomic<int> state{0};  // 0: being prepared, 1: ready, 2: preparation failed
+  std::exception_ptr error;
 };
 
 }  // namespace sqsg
@@ -559,51 +661,129 @@ struct Ticket {
 struct sqsg_server {
   sqsg_server_options opt{};
   std::unique_ptr<sqsg::BatchEngine> eng;
-  std::deque<uint64_t> queue;
-  std::vector<sqsg::Ticket> tickets;
+  std::deque<uint64_t> queue;          // submitted, not admitted (submission order)
+  std::deque<sqsg::Ticket> tickets;    // stable addresses: the preparation thread holds some
   std::map<sqsg_job *, uint64_t> active;
   size_t resident_bytes = 0;
   std::chrono::steady_clock::time_point t0;
+  // preparation thread: uploads submitted jobs' inputs while the steps run
+  int device = 0;
+  std::thread prep;
+  std::mutex m;
+  std::condition_variable cv;
+  std::deque<sqsg::Ticket *> prep_queue;
+  bool stopping = false;
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  ~sqsg_server() {
+    {
+      std::lock_guard<std::mutex> lock(m);
+      stopping = true;
+    }
+    cv.notify_all();
+    if (prep.joinable()) prep.join();
   }
 };
 
 namespace sqsg {
 
+namespace {
+
+void prepare_loop(sqsg_server *s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    Ticket *t = nullptr;
+    {
+      std::unique_lock<std::mutex> lock(s->m);
+      s->cv.wait(lock, [s] { return s->stopping || !s->prep_queue.empty(); });
+      if (s->stopping) return;
+      t = s->prep_queue.front();
+      s->prep_queue.pop_front();
+    }
+    try {
+      {
+        std::lock_guard<std::mutex> capture_lock(capture_mutex());
+        prepare_for_batch(t->job);
+      }
+      t->state.store(1, std::memory_order_release);
+    } catch (...) {
+      t->error = std::current_exception();
+      t->state.store(2, std::memory_order_release);
+    }
+  }
+}
+
+// device bytes a job holds while resident: its inputs and two record sinks
+size_t admission_bytes(const sqsg_job *job, uint64_t slice) {
+  size_t rec = 24 + 8 * job->problems.size() + job->stride;
+  return job_footprint(job) +
+         2 * rec * first_sink_records(job, uint64_t(device_cu_count()) * 8,
+                                      slice ? slice : default_launch_size(job));
+}
+
+}  // namespace
+
 sqsg_server *server_create(const sqsg_server_options &o) {
   auto s = std::make_unique<sqsg_server>();
   s->opt = o;
   if (!s->opt.max_active) s->opt.max_active = 64;
-  if (!s->opt.hbm_budget_bytes) s->opt.hbm_budget_bytes = size_t(8) << 30;
+  SQSG_HIP(hipGetDevice(&s->device));
+  if (!s->opt.hbm_budget_bytes) {  // most of the device memory that is free now
+    size_t free_b = 0, total_b = 0;
+    SQSG_HIP(hipMemGetInfo(&free_b, &total_b));
+    s->opt.hbm_budget_bytes = free_b / 10 * 8;
+  }
   s->eng = std::make_unique<BatchEngine>(true);
+  s->eng->probe_new_ = true;
   s->t0 = std::chrono::steady_clock::now();
+  s->prep = std::thread(prepare_loop, s.get());
   return s.release();
 }
 
 uint64_t server_submit(sqsg_server *s, sqsg_job *job) {
-  Ticket t;
+  s->tickets.emplace_back();
+  Ticket &t = s->tickets.back();
   t.job = job;
   t.t_submit = s->now();
-  s->tickets.push_back(t);
   const uint64_t id = s->tickets.size() - 1;
   s->queue.push_back(id);
+  {
+    std::lock_guard<std::mutex> lock(s->m);
+    s->prep_queue.push_back(&t);
+  }
+  s->cv.notify_one();
   return id;
 }
 
-// Admission by footprint, then one persistent step over the resident jobs.
+// Admission by footprint (in submission order, skipping jobs still being prepared or too
+// large for the memory left), then one persistent step over the resident jobs.
 void server_step(sqsg_server *s, int *active, int *queued) {
-  while (!s->queue.empty() && s->active.size() < s->opt.max_active) {
-    Ticket &t = s->tickets[s->queue.front()];
-    ensure_device(t.job);
-    const size_t need = job_footprint(t.job) + (size_t(1) << 14) * (24 + t.job->stride);
-    if (!s->active.empty() && s->resident_bytes + need > s->opt.hbm_budget_bytes) break;
+  for (auto it = s->queue.begin();
+       it != s->queue.end() && s->active.size() < s->opt.max_active;) {
+    Ticket &t = s->tickets[*it];
+    const int st = t.state.load(std::memory_order_acquire);
+    if (st == 0) {
+      ++it;
+      continue;
+    }
+    if (st == 2) {  // reported once: the ticket is done and the error is raised here
+      t.done = true;
+      t.t_done = s->now();
+      s->queue.erase(it);
+      std::rethrow_exception(t.error);
+    }
     if (s->opt.slice_candidates) t.job->launch_size = s->opt.slice_candidates;
+    const size_t need = admission_bytes(t.job, t.job->launch_size);
+    if (!s->active.empty() && s->resident_bytes + need > s->opt.hbm_budget_bytes) {
+      ++it;
+      continue;
+    }
     t.footprint = s->eng->add(t.job);
     s->resident_bytes += t.footprint;
     t.t_admit = s->now();
-    s->active[t.job] = s->queue.front();
-    s->queue.pop_front();
+    s->active[t.job] = *it;
+    it = s->queue.erase(it);
   }
   if (!s->eng->idle()) {
     for (sqsg_job *j : s->eng->step()) {

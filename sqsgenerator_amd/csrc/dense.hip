@@ -412,7 +412,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
     unsigned long long prune_key = ~0ull;
     if (p.prune) {
       const unsigned long long g0 =
-          __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          p.thr_key;
       const unsigned long long l0 = filter_bound(lf, p);
       prune_key = g0 < l0 ? g0 : l0;
     }
@@ -423,7 +423,7 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
     // a candidate above the threshold, or above the filter's L-th level when its turn
     // comes, cannot be inserted
     const unsigned long long g =
-        __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        p.thr_key;
     const bool below = active && okey(tot) <= g;
     const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
     if (pass) {
@@ -715,7 +715,7 @@ __device__ __forceinline__ void ws_record(const RP &p, const uint8_t *rows, uint
   if (am == 0) return;
   if (lane == 0) atomicAdd(p.scored, static_cast<unsigned long long>(__popcll(am)));
   const unsigned long long g =
-      __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      p.thr_key;
   const bool below = active && okey(tot) <= g;
   const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
   if (pass) {

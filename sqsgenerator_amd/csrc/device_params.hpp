@@ -1,6 +1,7 @@
 // Plain-old-data parameter blocks shared by the host driver and the HIP kernels.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 namespace sqsg {
@@ -94,9 +95,10 @@ struct RunParams {
   uint32_t off_levels;         // dense: the wavefront's tie filter levels (LevelFilter)
   uint32_t ws_pair;            // dense paired form: LDS bytes of one pair's region
   uint32_t prune;              // dense: stop a pass once no candidate can reach the kept levels
-  // the host's search threshold (order key of the keep-th distinct inserted objective as of
-  // the candidates merged before this launch; kernels only read it) and the record sink
-  unsigned long long *thr_key;
+  // the host's search threshold: order key of the keep-th distinct inserted objective as of
+  // the candidates merged before this launch was issued
+  unsigned long long thr_key;
+  // record sink and counters of the launch
   unsigned int *rec_count;
   uint64_t *rec_cand;
   double *rec_obj;
@@ -106,7 +108,18 @@ struct RunParams {
   uint32_t rec_stride;
   unsigned long long *scored;
   unsigned int *next_seg;      // dense single-job kernel: segment counter (zero at launch)
+  // persistent form: host memory (pinned, device-visible) that receives the record and
+  // scored counts and the first stage_cap records when the step ends, or null
+  uint8_t *stage;
+  uint32_t stage_cap;
 };
+
+// Layout of a job's stage: {record count u32, pad u32, scored u64} then `cap` candidates
+// (u64), objectives (f64), nprob sub-objectives each (f64) and species (rec_stride bytes
+// each).
+inline size_t stage_bytes(uint32_t cap, uint32_t nprob, uint32_t rec_stride) {
+  return 16 + size_t(cap) * (16 + 8 * nprob + rec_stride);
+}
 
 // One job's candidate range of a persistent step; wavefronts claim it `chunk` candidates
 // (one segment) at a time through the item's claim counter.

@@ -337,10 +337,18 @@ void ensure_device(sqsg_job *job) {
   for (auto &P : job->problems) P.upload_device();
   if (job->lane_ok)
     for (auto &P : job->problems) P.build_lane(lane_field_bits(job->kmax));
-  SQSG_HIP(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking));
-  SQSG_HIP(hipEventCreate(&job->ev0));
-  SQSG_HIP(hipEventCreate(&job->ev1));
   job->on_device = true;
+}
+
+// The job's own stream and timing events, created on first use: jobs that only run in the
+// batched or served form never hold one (a process has few hardware queues to share).
+hipStream_t job_stream(sqsg_job *job) {
+  if (!job->stream) {
+    SQSG_HIP(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking));
+    SQSG_HIP(hipEventCreate(&job->ev0));
+    SQSG_HIP(hipEventCreate(&job->ev1));
+  }
+  return job->stream;
 }
 
 DevProblem Problem::dev() const {
@@ -738,9 +746,8 @@ void RecordSink::bind(RunParams &p) const {
   p.rec_species = species;
 }
 
-void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob,
-                       const std::vector<std::pair<uint64_t, uint64_t>> &ranges,
-                       hipStream_t s) {
+std::shared_ptr<RecordBlock> RecordSink::fetch(const sqsg_job *job, uint32_t count,
+                                               uint32_t nprob, hipStream_t s) const {
   const uint32_t stride = job->stride;
   auto blk = std::make_shared<RecordBlock>();
   blk->cand.resize(count);
@@ -758,9 +765,21 @@ void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob,
                             hipMemcpyDeviceToHost, s));
     SQSG_HIP(hipStreamSynchronize(s));
   }
+  return blk;
+}
+
+void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob,
+                       const std::vector<std::pair<uint64_t, uint64_t>> &ranges,
+                       hipStream_t s) {
+  merge_block(job, fetch(job, count, nprob, s), ranges);
+}
+
+void merge_block(sqsg_job *job, std::shared_ptr<const RecordBlock> blk,
+                 const std::vector<std::pair<uint64_t, uint64_t>> &ranges) {
   // the threshold only decreases as candidates are merged, so a record above it now is
   // above it at its turn: it is dropped before the sort
   const double search = job->coll.search;
+  const uint32_t count = static_cast<uint32_t>(blk->cand.size());
   for (const auto &[begin, end] : ranges) {
     HostRecords r;
     r.begin = begin;
@@ -772,6 +791,21 @@ void RecordSink::drain(sqsg_job *job, uint32_t count, uint32_t nprob,
     std::sort(r.order.begin(), r.order.end());
     merge_records(job, std::move(r));
   }
+}
+
+std::shared_ptr<RecordBlock> stage_block(const sqsg_job *job, const uint8_t *stage,
+                                         uint32_t cap, uint32_t count, uint32_t nprob) {
+  const uint32_t stride = job->stride;
+  auto blk = std::make_shared<RecordBlock>();
+  const uint64_t *cand = reinterpret_cast<const uint64_t *>(stage + 16);
+  const double *obj = reinterpret_cast<const double *>(cand + cap);
+  const double *sub = obj + cap;
+  const uint8_t *sp = reinterpret_cast<const uint8_t *>(sub + size_t(cap) * nprob);
+  blk->cand.assign(cand, cand + count);
+  blk->obj.assign(obj, obj + count);
+  blk->sub.assign(sub, sub + size_t(count) * nprob);
+  blk->species.assign(sp, sp + size_t(count) * stride);
+  return blk;
 }
 
 uint64_t default_launch_size(const sqsg_job *job) {
@@ -806,6 +840,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user, const FixedScratch 
                                 "(a first sqsg_run of the job with this kernel uploads it)");
   begin_run(job);
   if (dense) ensure_dense(job);
+  const hipStream_t stream = job_stream(job);
   const uint32_t W = variant == SQSG_KERNEL_SPARSE_WAVE ? 4 : 1;
   uint32_t blocks = 0;
   RunParams p = make_run_params(job, variant, W, &blocks);
@@ -815,7 +850,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user, const FixedScratch 
   RecordSink sink;
   const bool ramp_up = !records_bounded(job);
   DevBuf d_ctl;
-  uint8_t *ctl = nullptr;  // threshold key, record count, scored count, segment counter
+  uint8_t *ctl = nullptr;  // record count (+8), scored count (+16), segment counter (+24)
   if (fixed) {
     if (sink.place(static_cast<uint8_t *>(fixed->base) + 256, fixed->bytes - 256, stride,
                    p.nprob) == 0)
@@ -826,9 +861,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user, const FixedScratch 
     d_ctl.alloc(256);
     ctl = d_ctl.as<uint8_t>();
   }
-  unsigned long long thr = search_key(job);
-  SQSG_HIP(hipMemcpy(ctl, &thr, sizeof(thr), hipMemcpyHostToDevice));
-  p.thr_key = reinterpret_cast<unsigned long long *>(ctl);
+  p.thr_key = search_key(job);
   p.rec_count = reinterpret_cast<unsigned int *>(ctl + 8);
   p.scored = reinterpret_cast<unsigned long long *>(ctl + 16);
   p.next_seg = reinterpret_cast<unsigned int *>(ctl + 24);
@@ -855,17 +888,17 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user, const FixedScratch 
     p.cand_begin = begin;
     p.cand_end = end;
     sink.bind(p);
-    SQSG_HIP(hipMemsetAsync(ctl + 8, 0, 24, job->stream));  // record, scored, segment counts
-    SQSG_HIP(hipEventRecord(job->ev0, job->stream));
-    SQSG_HIP(launch_variant(p, job, variant, blocks, W, job->stream));
-    SQSG_HIP(hipEventRecord(job->ev1, job->stream));
+    SQSG_HIP(hipMemsetAsync(ctl + 8, 0, 24, stream));  // record, scored, segment counts
+    SQSG_HIP(hipEventRecord(job->ev0, stream));
+    SQSG_HIP(launch_variant(p, job, variant, blocks, W, stream));
+    SQSG_HIP(hipEventRecord(job->ev1, stream));
     unsigned int count = 0;
     unsigned long long scored = 0;
     SQSG_HIP(hipMemcpyAsync(&count, p.rec_count, sizeof(count), hipMemcpyDeviceToHost,
-                            job->stream));
+                            stream));
     SQSG_HIP(hipMemcpyAsync(&scored, p.scored, sizeof(scored), hipMemcpyDeviceToHost,
-                            job->stream));
-    SQSG_HIP(hipStreamSynchronize(job->stream));
+                            stream));
+    SQSG_HIP(hipStreamSynchronize(stream));
     float ms = 0;
     SQSG_HIP(hipEventElapsedTime(&ms, job->ev0, job->ev1));
     job->kernel_ms += ms;
@@ -882,12 +915,8 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user, const FixedScratch 
     }
     if (scored != end - begin) throw DeviceError("device scored an unexpected candidate count");
     job->scored += scored;
-    sink.drain(job, count, p.nprob, {{begin, end}}, job->stream);
-    const unsigned long long hk = search_key(job);
-    if (hk != thr) {
-      thr = hk;
-      SQSG_HIP(hipMemcpy(ctl, &thr, sizeof(thr), hipMemcpyHostToDevice));
-    }
+    sink.drain(job, count, p.nprob, {{begin, end}}, stream);
+    p.thr_key = search_key(job);
     job->stats.finished = job->scored;
     begin = end;
     ++ramp;
@@ -929,7 +958,7 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     sp.off_hist = align(64 * (P.N + 1));
     sp.wave_lds = align(sp.off_hist + job->kmax * ((job->kmax + 1) / 2) * 256);
     if (sp.wave_lds > 160 * 1024) throw std::out_of_range("problem exceeds LDS");
-    SQSG_HIP(launch_score_lane(sp, job->kmax, s ? s : job->stream));
+    SQSG_HIP(launch_score_lane(sp, job->kmax, s ? s : job_stream(job)));
     return;
   }
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
@@ -956,7 +985,7 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     sp.off_row = rp.off_rows[0];
     sp.off_hist = rp.off_hist;
     sp.wave_lds = rp.wave_lds;
-    SQSG_HIP(launch_score_dense(sp, dense_tile_bound_of(job), s ? s : job->stream));
+    SQSG_HIP(launch_score_dense(sp, dense_tile_bound_of(job), s ? s : job_stream(job)));
     return;
   }
   if (variant != SQSG_KERNEL_SPARSE_WAVE)
@@ -966,7 +995,7 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
   sp.off_terms = align(sp.off_hist + 4 * P.S * P.K * P.K);
   sp.wave_lds = align(sp.off_terms + 8 * static_cast<uint32_t>(P.terms.size()));
   if (size_t(4) * sp.wave_lds > 160 * 1024) throw std::out_of_range("problem exceeds LDS");
-  SQSG_HIP(launch_score_explicit(sp, s ? s : job->stream));
+  SQSG_HIP(launch_score_explicit(sp, s ? s : job_stream(job)));
 }
 
 }  // namespace sqsg

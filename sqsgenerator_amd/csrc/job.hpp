@@ -106,8 +106,11 @@ struct LevelRecords {
   std::vector<uint64_t> cand;
   std::vector<double> sub;        // one objective per problem and record
   std::vector<uint8_t> species;   // sorted-site packed species, `stride` bytes per record
-  std::unordered_multimap<uint64_t, uint32_t> seen;  // species hash -> record (Monte Carlo)
+  // Monte Carlo: open-addressing set of the level's configurations (hash, record + 1)
+  std::vector<std::pair<uint64_t, uint32_t>> seen;
   size_t size() const { return cand.size(); }
+  // inserts the configuration at `sp` (stride bytes, hash h) unless the level holds it
+  bool insert_unique(const uint8_t *sp, uint32_t stride, uint64_t h);
 };
 
 // One copy of a record sink's contents on the host.
@@ -190,6 +193,14 @@ struct sqsg_job {
 
 namespace sqsg {
 
+// merges the records of a host copy of a sink, split by candidate range, into the job's
+// collection in candidate order
+void merge_block(sqsg_job *job, std::shared_ptr<const RecordBlock> blk,
+                 const std::vector<std::pair<uint64_t, uint64_t>> &ranges);
+// host copy of the first `count` records of a persistent step in a job's host stage
+// (RunParams::stage layout, room for `cap` records)
+std::shared_ptr<RecordBlock> stage_block(const sqsg_job *job, const uint8_t *stage,
+                                         uint32_t cap, uint32_t count, uint32_t nprob);
 // Device memory a run uses instead of allocating its own: control words and the record sink.
 struct FixedScratch {
   void *base = nullptr;
@@ -221,6 +232,9 @@ struct RecordSink {
   // its capacity in records
   uint32_t place(void *base, size_t bytes, uint32_t stride, uint32_t nprob);
   void bind(RunParams &p) const;
+  // copies `count` records back to the host
+  std::shared_ptr<RecordBlock> fetch(const sqsg_job *job, uint32_t count, uint32_t nprob,
+                                     hipStream_t s) const;
   // copies `count` records of the candidate ranges back and merges them into the job's
   // collection in candidate order
   void drain(sqsg_job *job, uint32_t count, uint32_t nprob,
@@ -251,6 +265,8 @@ void result_species_z(const sqsg_job *job, uint64_t r, uint8_t *out);
 std::string result_rank(const sqsg_job *job, uint64_t r);
 bool result_rank_u64(const sqsg_job *job, uint64_t r, uint64_t *rank);
 void result_sro(sqsg_job *job, uint64_t first, uint64_t count, double *out);
+// re-scores and keeps the SRO cubes of every result (packs of up to 2^20 results)
+void cache_sro(sqsg_job *job);
 double result_objective(const sqsg_job *job, uint64_t r);
 void result_bulk(sqsg_job *job, uint64_t first, uint64_t count, double *objective,
                  uint64_t *candidate, uint8_t *species_z, uint64_t *rank);
@@ -263,6 +279,8 @@ void server_step(sqsg_server *s, int *active, int *queued);
 void server_poll(sqsg_server *s, uint64_t id, int *done, double *ts, double *ta, double *td);
 void server_destroy(sqsg_server *s);
 void ensure_device(sqsg_job *job);
+// the job's own stream (created on first use)
+hipStream_t job_stream(sqsg_job *job);
 void ensure_dense(sqsg_job *job);
 // dense variant launch shape of a job: tile bound (kernel template) and 1-wave blocks
 uint32_t dense_tile_bound_of(const sqsg_job *job);

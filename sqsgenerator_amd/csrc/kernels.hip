@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void run_wave_kernel(RunParams p) {
         ++scored;
         const unsigned long long key = okey(tot);
         const unsigned long long g =
-            __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p.thr_key;
         // the wave's L smallest distinct objectives of its earlier candidates: a candidate
         // above the L-th has L smaller distinct objectives before it and is not inserted
         const unsigned long long lk =

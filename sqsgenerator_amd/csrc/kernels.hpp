@@ -25,6 +25,8 @@ hipError_t launch_score_lane(const ScoreParams &p, uint32_t kmax, hipStream_t st
 hipError_t launch_batch_lane(const BatchParams &bp, uint32_t kmax, uint32_t blocks,
                              uint32_t waves_per_block, hipStream_t stream);
 hipError_t launch_batch_reset(const BatchParams &bp, uint32_t max_jobs, hipStream_t stream);
+// copies every job's counters and first records into its host stage (RunParams::stage)
+hipError_t launch_batch_stage(const BatchParams &bp, uint32_t max_jobs, hipStream_t stream);
 // raises the dynamic LDS limit of the persistent kernel (called outside graph capture)
 hipError_t prepare_batch_lane(uint32_t kmax, size_t lds);
 // bits per packed neighbour-count field for a largest species count kmax

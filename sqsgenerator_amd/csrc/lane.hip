@@ -170,7 +170,7 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
     }
     scored += static_cast<unsigned long long>(__popcll(__ballot(active)));
     const unsigned long long g =
-        __hip_atomic_load(p.thr_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        p.thr_key;
     // a candidate above the threshold, or above the filter's L-th level when its turn
     // comes, cannot be inserted
     const bool below = active && okey(tot) <= g;
@@ -246,6 +246,34 @@ __global__ __launch_bounds__(256) void batch_lane_kernel(BatchParams bp) {
   if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
 }
 
+// After a persistent step: copies each job's record and scored counts and its first
+// stage_cap records into the job's stage in host memory (one block per job, coalesced
+// writes), so the host reads them without a copy that would wait for the next step's
+// kernel.
+__global__ __launch_bounds__(256) void batch_stage_kernel(BatchParams bp) {
+  const uint32_t t = blockIdx.x;
+  if (t >= *bp.num_jobs) return;
+  const RunParams &p = bp.jobs[t];
+  if (!p.stage) return;
+  const unsigned int count = *p.rec_count;
+  const uint32_t n = min(min(count, p.rec_cap), p.stage_cap);
+  uint8_t *st = p.stage;
+  if (threadIdx.x == 0) {
+    *reinterpret_cast<unsigned int *>(st) = count;
+    *reinterpret_cast<unsigned long long *>(st + 8) = *p.scored;
+  }
+  uint64_t *cand = reinterpret_cast<uint64_t *>(st + 16);
+  double *obj = reinterpret_cast<double *>(cand + p.stage_cap);
+  double *sub = obj + p.stage_cap;
+  uint8_t *sp = reinterpret_cast<uint8_t *>(sub + static_cast<size_t>(p.stage_cap) * p.nprob);
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+    cand[k] = p.rec_cand[k];
+    obj[k] = p.rec_obj[k];
+  }
+  for (uint32_t k = threadIdx.x; k < n * p.nprob; k += blockDim.x) sub[k] = p.rec_sub[k];
+  for (uint32_t k = threadIdx.x; k < n * p.rec_stride; k += blockDim.x) sp[k] = p.rec_species[k];
+}
+
 // zeroes the item claim counters and every job's record and scored counters (a kernel, so
 // the reset can sit inside a captured graph)
 __global__ void batch_reset_kernel(BatchParams bp, uint32_t max_jobs) {
@@ -318,6 +346,11 @@ template <int KM> hipError_t launch_batch_lane_k(const BatchParams &bp, uint32_t
 }
 
 }  // namespace
+
+hipError_t launch_batch_stage(const BatchParams &bp, uint32_t max_jobs, hipStream_t s) {
+  hipLaunchKernelGGL(batch_stage_kernel, dim3(max_jobs), dim3(256), 0, s, bp);
+  return hipGetLastError();
+}
 
 hipError_t launch_batch_reset(const BatchParams &bp, uint32_t max_jobs, hipStream_t s) {
   const uint32_t n = max_jobs > bp.max_items ? max_jobs : bp.max_items;

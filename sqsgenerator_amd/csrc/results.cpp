@@ -27,10 +27,20 @@ namespace sqsg {
 
 namespace {
 
+// 64-bit words mixed by multiply and shift (collisions are resolved by comparing bytes)
 uint64_t hash_bytes(const uint8_t *p, size_t n) {
-  uint64_t h = 0xcbf29ce484222325ull;
-  for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 0x100000001b3ull;
-  return h ^ (h >> 31);
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ n;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, p + i, 8);
+    h = (h ^ w) * 0xbf58476d1ce4e5b9ull;
+    h ^= h >> 31;
+  }
+  uint64_t w = 0;
+  std::memcpy(&w, p + i, n - i);
+  h = (h ^ w) * 0x94d049bb133111ebull;
+  return h ^ (h >> 29);
 }
 
 // 128-bit restatement of rank_permutation (permutation.h:34-50); exact whenever
@@ -113,14 +123,8 @@ void merge_in_order(sqsg_job *job, const HostRecords &r) {
     const bool fresh = it == C.levels.end();
     if (fresh) it = C.levels.emplace(o, LevelRecords{}).first;
     LevelRecords &lv = it->second;
-    if (!systematic) {  // a configuration repeats only in Monte Carlo runs
-      const uint64_t h = hash_bytes(sp, stride);
-      bool dup = false;
-      for (auto [a, b] = lv.seen.equal_range(h); a != b && !dup; ++a)
-        dup = std::memcmp(lv.species.data() + size_t(a->second) * stride, sp, stride) == 0;
-      if (dup) continue;
-      lv.seen.emplace(h, static_cast<uint32_t>(lv.size()));
-    }
+    // a configuration repeats only in Monte Carlo runs
+    if (!systematic && !lv.insert_unique(sp, stride, hash_bytes(sp, stride))) continue;
     lv.cand.push_back(t);
     lv.sub.insert(lv.sub.end(), B.sub.begin() + size_t(k) * nprob,
                   B.sub.begin() + size_t(k + 1) * nprob);
@@ -138,6 +142,30 @@ void merge_in_order(sqsg_job *job, const HostRecords &r) {
 }
 
 }  // namespace
+
+bool LevelRecords::insert_unique(const uint8_t *sp, uint32_t stride, uint64_t h) {
+  if (2 * (size() + 1) > seen.size()) {  // grow (load at most one half), rehash
+    size_t cap = 64;
+    while (cap < 4 * (size() + 1)) cap *= 2;  // a power of two: slots are hash & (cap - 1)
+    std::vector<std::pair<uint64_t, uint32_t>> old(cap, {0, 0});
+    old.swap(seen);
+    const size_t mask = seen.size() - 1;
+    for (const auto &e : old)
+      if (e.second) {
+        size_t k = e.first & mask;
+        while (seen[k].second) k = (k + 1) & mask;
+        seen[k] = e;
+      }
+  }
+  const size_t mask = seen.size() - 1;
+  size_t k = h & mask;
+  for (; seen[k].second; k = (k + 1) & mask)
+    if (seen[k].first == h &&
+        std::memcmp(species.data() + size_t(seen[k].second - 1) * stride, sp, stride) == 0)
+      return false;
+  seen[k] = {h, static_cast<uint32_t>(size() + 1)};
+  return true;
+}
 
 unsigned long long merge_records(sqsg_job *job, HostRecords &&recs) {
   Collection &C = job->coll;
@@ -169,12 +197,8 @@ void finalize_results(sqsg_job *job, bool eager_sro) {
   }
   job->coll.clear();
   job->coll.search = DBL_MAX;
-  // SRO cubes are re-scored on the device: eagerly for packs of ordinary size
-  if (eager_sro && R.size() && R.size() <= (uint64_t(1) << 20)) {
-    std::vector<double> sro(R.size() * job->cube_total);
-    result_sro(job, 0, R.size(), sro.data());
-    R.sro = std::move(sro);
-  }
+  // SRO cubes are re-scored on the device, now or on first request
+  if (eager_sro) cache_sro(job);
 }
 
 double result_objective(const sqsg_job *job, uint64_t r) {
@@ -239,14 +263,12 @@ bool result_rank_u64(const sqsg_job *job, uint64_t r, uint64_t *rank) {
   return ok;
 }
 
-void result_sro(sqsg_job *job, uint64_t first, uint64_t count, double *out) {
+namespace {
+
+// SRO cubes of results [first, first + count), re-scored on the device; each re-scored
+// objective must equal the recorded one
+void rescore_sro(sqsg_job *job, uint64_t first, uint64_t count, double *out) {
   const ResultSet &R = job->res;
-  if (first + count > R.size()) throw std::out_of_range("result index out of range");
-  if (!R.sro.empty()) {
-    std::memcpy(out, R.sro.data() + first * job->cube_total,
-                count * job->cube_total * sizeof(double));
-    return;
-  }
   const uint64_t batch = uint64_t(1) << 20;
   for (uint64_t b0 = 0; b0 < count; b0 += batch) {
     const uint64_t B = std::min(batch, count - b0);
@@ -260,13 +282,13 @@ void result_sro(sqsg_job *job, uint64_t first, uint64_t count, double *out) {
       DevBuf d_sp(sp.size()), d_obj(B * sizeof(double)), d_sro(B * nb * sizeof(double));
       SQSG_HIP(hipMemcpy(d_sp.as<void>(), sp.data(), sp.size(), hipMemcpyHostToDevice));
       score_batch(job, q, d_sp.as<uint8_t>(), static_cast<uint32_t>(B), nullptr,
-                  d_obj.as<double>(), d_sro.as<double>(), SQSG_KERNEL_SPARSE_WAVE, job->stream);
+                  d_obj.as<double>(), d_sro.as<double>(), SQSG_KERNEL_SPARSE_WAVE, job_stream(job));
       std::vector<double> obj(B), sro(B * nb);
       SQSG_HIP(hipMemcpyAsync(obj.data(), d_obj.as<void>(), B * sizeof(double),
-                              hipMemcpyDeviceToHost, job->stream));
+                              hipMemcpyDeviceToHost, job_stream(job)));
       SQSG_HIP(hipMemcpyAsync(sro.data(), d_sro.as<void>(), sro.size() * sizeof(double),
-                              hipMemcpyDeviceToHost, job->stream));
-      SQSG_HIP(hipStreamSynchronize(job->stream));
+                              hipMemcpyDeviceToHost, job_stream(job)));
+      SQSG_HIP(hipStreamSynchronize(job_stream(job)));
       const uint32_t nprob = static_cast<uint32_t>(job->problems.size());
       for (uint64_t k = 0; k < B; ++k) {
         if (obj[k] != R.sub[(first + b0 + k) * nprob + q])
@@ -278,6 +300,28 @@ void result_sro(sqsg_job *job, uint64_t first, uint64_t count, double *out) {
       cube_off += nb;
     }
   }
+}
+
+}  // namespace
+
+void cache_sro(sqsg_job *job) {
+  ResultSet &R = job->res;
+  if (!R.sro.empty() || !R.size() || R.size() > (uint64_t(1) << 20)) return;
+  std::vector<double> sro(R.size() * job->cube_total);
+  rescore_sro(job, 0, R.size(), sro.data());
+  R.sro = std::move(sro);
+}
+
+void result_sro(sqsg_job *job, uint64_t first, uint64_t count, double *out) {
+  const ResultSet &R = job->res;
+  if (first + count > R.size()) throw std::out_of_range("result index out of range");
+  cache_sro(job);  // packs of ordinary size: every cube at once, kept
+  if (!R.sro.empty()) {
+    std::memcpy(out, R.sro.data() + first * job->cube_total,
+                count * job->cube_total * sizeof(double));
+    return;
+  }
+  rescore_sro(job, first, count, out);
 }
 
 }  // namespace sqsg
