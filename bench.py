@@ -220,6 +220,53 @@ def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
     return out
 
 
+def batched_uneven(iterations: int) -> dict:
+    """configurations (a)-(d): FCC 108 ternary, BCC 128 quaternary, HCP 128 binary and the
+    rocksalt 216 split job, `iterations` MC candidates each, in one batched run (inputs
+    resident before the clock starts); each job's batched rate alone beside it"""
+    import sqsgenerator_amd._lib as L
+    import workloads as W
+    from sqsgenerator_amd.core import Job, batch_run, parse_config
+
+    makers = [("a_fcc108_ternary_3shells", W.ternary_108),
+              ("b_bcc128_quaternary_4shells", W.quaternary_bcc_128),
+              ("c_hcp128_binary_5shells", W.hcp_128),
+              ("d_rocksalt216_split_3+3shells", W.rocksalt_split_216)]
+
+    def make(fn):
+        job = Job(parse_config(fn(iterations=iterations)))
+        for q in range(job.num_problems()):  # upload inputs (and the dense tile stream)
+            job.score(job.problem(q)["species_packed"][None, :], q,
+                      kernel=L.KERNEL_DENSE_MFMA if job.kernel_shape().startswith("dense")
+                      else L.KERNEL_SPARSE_LANE)
+        return job
+
+    def timed(jobs):
+        L.check(L.lib().sqsg_device_synchronize())
+        t0 = time.perf_counter()
+        rep = batch_run(jobs)
+        L.check(L.lib().sqsg_device_synchronize())
+        return time.perf_counter() - t0, rep
+
+    out = {"iterations_per_job": iterations, "jobs": []}
+    for name, fn in makers:
+        job = make(fn)
+        wall, rep = timed([job])
+        out["jobs"].append({"job": name, "shape": job.kernel_shape(),
+                            "alone_candidates_per_s": rep["candidates"] / wall,
+                            "alone_kernel_ms": rep["kernel_ms"], "alone_steps": rep["launches"]})
+        job.close()
+    jobs = [make(fn) for _, fn in makers]
+    wall, rep = timed(jobs)
+    for j in jobs:
+        j.close()
+    alone_s = sum(iterations / j["alone_candidates_per_s"] for j in out["jobs"])
+    out.update({"together_candidates_per_s": rep["candidates"] / wall, "together_wall_s": wall,
+                "together_kernel_ms": rep["kernel_ms"], "together_steps": rep["launches"],
+                "sum_of_alone_wall_s": alone_s})
+    return out
+
+
 def serving_jobs(n, seed, max_budget):
     """request mix of the serving stream: FCC/BCC/HCP cells of 32-864 sites, 2-6 species
     with random integer compositions, 2-7 shells (weights 1/s), log-uniform budgets"""
@@ -317,6 +364,9 @@ def main() -> int:
                     help="only the large-cell sparse-vs-dense scoring (4096 candidates per launch)")
     ap.add_argument("--large-cell-launches", type=int, default=10)
     ap.add_argument("--no-large-cells", action="store_true")
+    ap.add_argument("--uneven-iterations", type=int, default=10_000_000,
+                    help="MC candidates per job of the batched uneven leg (configs (a)-(d)); "
+                         "0: skip it")
     ap.add_argument("--no-prune", action="store_true",
                     help="skip the secondary run with early exit (sqsg_job_set_pruning)")
     ap.add_argument("--serve-sweep", default="10,30,100,300,1000",
@@ -457,6 +507,8 @@ def main() -> int:
     }
     if not args.no_large_cells:
         out["large_cell_scoring"] = large_cell_scoring(args.large_cell_launches)
+    if args.uneven_iterations:
+        out["batched_uneven"] = batched_uneven(args.uneven_iterations)
     if not args.no_serve:
         out["serving"] = serving(args.serve_rate, args.serve_seconds, 20261203,
                                  args.serve_max_budget)

@@ -117,6 +117,7 @@ struct Group {
   uint32_t *h_counts[2] = {nullptr, nullptr};
   uint32_t graph_wave_lds = 0, graph_blocks = 0;
   double work = 0;  // estimated cost of the current step (candidates x pair work)
+  bool split = false;  // dense: the kernel form of split-mode jobs
   std::vector<Slot *> members;
   Group() {
     for (int q = 0; q < 2; ++q) {
@@ -206,7 +207,9 @@ class BatchEngine {
     s->chunk = gen * s->p.rounds * (variant == SQSG_KERNEL_DENSE_PAIRED ? kDensePairs : 1);
     if (dense) {  // every grid position of the tile bound costs its MFMAs in every shell
       const double TT = dense_tile_bound_of(job);
-      s->cost = job->problems[0].S * TT * (TT + 1) / 2 / s->p.dense_cw;
+      double shells = 0;
+      for (const auto &P : job->problems) shells += P.S;
+      s->cost = shells * TT * (TT + 1) / 2 / s->p.dense_cw;
     } else {
       s->cost = static_cast<double>(work) / 64;
     }
@@ -231,12 +234,14 @@ class BatchEngine {
     s->p.thr_key = search_key(job);  // travels in the job table of every step
     s->footprint = footprint(*s);
     // groups share a launch shape: the tile bound (dense), also K (paired: one LDS
-    // layout), the largest species count (lane)
+    // layout), the largest species count (lane); split jobs have dense kernels of their own
+    const bool split = job->problems.size() > 1;
     const uint32_t key = !dense ? job->kmax
                          : variant == SQSG_KERNEL_DENSE_PAIRED
                              ? dense_tile_bound_of(job) | (job->problems[0].K << 8)
-                             : dense_tile_bound_of(job);
+                             : dense_tile_bound_of(job) | (split ? 1u << 16 : 0u);
     Group &g = group(variant, key, dense ? dense_tile_bound_of(job) : job->kmax);
+    g.split = dense && split;
     if (g.members.size() >= kGroupJobs) throw std::out_of_range("too many resident jobs");
     g.members.push_back(s.get());
     s->group = &g;
@@ -489,7 +494,7 @@ class BatchEngine {
     uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / std::max<size_t>(block_lds, 1));
     per_cu = std::max(1u, std::min(per_cu, 16u));
     if (g.variant == SQSG_KERNEL_DENSE_MFMA)
-      per_cu = dense_waves_per_cu(g.shape, block_lds, true);
+      per_cu = dense_waves_per_cu(g.shape, block_lds, true, g.split);
     else if (g.variant == SQSG_KERNEL_DENSE_PAIRED)
       per_cu = dense_paired_blocks_per_cu(g.shape, block_lds, true);
     const uint32_t blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
@@ -503,7 +508,7 @@ class BatchEngine {
     bp.claims = g.d_claims.as<unsigned int>();
     bp.max_items = kGroupItems;
     g.blocks = blocks;
-    SQSG_HIP(g.variant == SQSG_KERNEL_DENSE_MFMA     ? prepare_batch_dense(g.shape, block_lds)
+    SQSG_HIP(g.variant == SQSG_KERNEL_DENSE_MFMA     ? prepare_batch_dense(g.shape, block_lds, g.split)
              : g.variant == SQSG_KERNEL_DENSE_PAIRED ? prepare_batch_dense_paired(g.shape, block_lds)
                                                      : prepare_batch_lane(g.shape, block_lds));
     if (use_graph_ && (!g.exec || g.graph_wave_lds != wave_lds || g.graph_blocks != blocks)) {
@@ -533,7 +538,8 @@ class BatchEngine {
 
   hipError_t launch_kernel(const Group &g, hipStream_t st) {
     switch (g.variant) {
-      case SQSG_KERNEL_DENSE_MFMA: return launch_batch_dense(g.bp, g.shape, g.blocks, st);
+      case SQSG_KERNEL_DENSE_MFMA:
+        return launch_batch_dense(g.bp, g.shape, g.blocks, st, g.split);
       case SQSG_KERNEL_DENSE_PAIRED:
         return launch_batch_dense_paired(g.bp, g.shape, g.blocks, st);
       default: return launch_batch_lane(g.bp, g.shape, g.blocks, 1, st);

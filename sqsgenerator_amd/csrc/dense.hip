@@ -28,6 +28,8 @@
 // overlap the row-end packing (accumulator reads, byte packing) with the next row's MFMAs.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "candidates.hpp"
 #include "kernels.hpp"
 
@@ -440,7 +442,66 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
   }
 }
 
-template <int TT, int G, int W, int D>
+// One segment of a split-mode job (sqs.h:404-418): a candidate holds one configuration per
+// sublattice problem, each shuffled with its own stream (split mode shuffles every candidate,
+// sqs.h:407); the problems share the kernel's tile bound and the pass width of the largest
+// species count, and are scored one after the other.  The objective is 0.0 plus the
+// problems' objectives in order (helpers::sum folds from T(0), core/helpers/fold.h:51-53);
+// each lane's per-problem objectives wait in LDS (off_obj, [problem][lane]).
+template <int TT, int G, int D, class RP>
+__device__ __forceinline__ void dense_segment_split(const RP &p, uint64_t sbeg, uint64_t end,
+                                                    uint8_t *wl, int lane, LevelFilter &lf,
+                                                    unsigned long long &scored) {
+  const uint32_t GEN = p.gen_group, CW = p.dense_cw, stride = p.dense_stride;
+  const uint32_t npass = GEN / CW, nprob = p.nprob;
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(wl + p.off_hist);
+  double *subs = reinterpret_cast<double *>(wl + p.off_obj);
+  const bool gl = static_cast<uint32_t>(lane) < GEN;
+  const uint64_t cand = sbeg + static_cast<uint64_t>(lane);  // Monte Carlo: one round
+  const bool active = gl && cand < end;
+  for (uint32_t q = 0; q < nprob; ++q) {
+    uint8_t *row = wl + p.off_rows[q] + (gl ? static_cast<uint32_t>(lane) : 0u) * stride;
+    if (gl) {
+      load_dense_row<TT>(row, p.prob[q]);
+      if (active) shuffle_candidate(Strided<0>{row}, p.prob[q], cand);
+    }
+  }
+  const unsigned long long am = __ballot(active);
+  if (am == 0) return;
+  unsigned long long live = 0;
+  for (uint32_t q = 0; q < npass; ++q)
+    if ((am >> (q * CW)) & low_mask(CW)) live |= 1ull << q;
+  wave_sync();
+  double tot = 0.0;
+  for (uint32_t q = 0; q < nprob; ++q) {
+    const double o = dense_score<TT, G, D, kDenseTermTables<TT>>(
+        p.prob[q], wl + p.off_rows[q], stride, cnt, CW, npass, live, lane, nullptr, nullptr, 0u,
+        am, ~0ull);
+    subs[q * 64 + static_cast<uint32_t>(lane)] = o;
+    tot += o;
+  }
+  scored += static_cast<unsigned long long>(__popcll(am));
+  const unsigned long long g = p.thr_key;
+  const bool below = active && okey(tot) <= g;
+  const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
+  if (pass) {
+    const uint32_t slot = atomicAdd(p.rec_count, 1u);
+    if (slot < p.rec_cap) {
+      uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
+      for (uint32_t q = 0; q < nprob; ++q) {
+        const uint8_t *row = wl + p.off_rows[q] + static_cast<uint32_t>(lane) * stride;
+        for (uint32_t i = 0; i < p.prob[q].N; ++i) dst[i] = row[i];
+        dst += p.prob[q].N;
+        p.rec_sub[slot * nprob + q] = subs[q * 64 + static_cast<uint32_t>(lane)];
+      }
+      p.rec_cand[slot] = cand;
+      p.rec_obj[slot] = tot;
+    }
+  }
+  wave_sync();  // rows are rewritten by the next segment
+}
+
+template <int TT, int G, int W, int D, bool SPLIT>
 __global__ __launch_bounds__(64, W) void run_dense_kernel(RunParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
@@ -458,14 +519,19 @@ __global__ __launch_bounds__(64, W) void run_dense_kernel(RunParams p) {
     if (lane == 0) seg = atomicAdd(p.next_seg, 1u);
     seg = __builtin_amdgcn_readfirstlane(__shfl(seg, 0));
     if (seg >= nseg) break;
-    dense_segment<TT, G, D>(p, p.cand_begin + seg * per_seg, p.cand_end, lds, lane, lf, scored);
+    if constexpr (SPLIT)
+      dense_segment_split<TT, G, D>(p, p.cand_begin + seg * per_seg, p.cand_end, lds, lane, lf,
+                                    scored);
+    else
+      dense_segment<TT, G, D>(p, p.cand_begin + seg * per_seg, p.cand_end, lds, lane, lf,
+                              scored);
   }
   if (lane == 0 && scored) atomicAdd(p.scored, scored);
 }
 
 // Persistent form: every wavefront claims one-segment chunks of the jobs' candidate ranges
 // until the work list is drained; the jobs of a launch share the tile bound TT.
-template <int TT, int G, int W, int D>
+template <int TT, int G, int W, int D, bool SPLIT>
 __global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
@@ -486,7 +552,10 @@ __global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
       lf->n = 0;
       last_job = job;
     }
-    dense_segment<TT, G, D>(p, cbeg, cend, lds, lane, *lf, scored);
+    if constexpr (SPLIT)
+      dense_segment_split<TT, G, D>(p, cbeg, cend, lds, lane, *lf, scored);
+    else
+      dense_segment<TT, G, D>(p, cbeg, cend, lds, lane, *lf, scored);
   }
   if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
 }
@@ -882,20 +951,44 @@ hipError_t dense_lds(const void *fn, size_t bytes) {
                              static_cast<int>(bytes));
 }
 
-template <class Sh> hipError_t run_dense_k(const RunParams &p, uint32_t blocks, hipStream_t s) {
-  const void *fn = reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>);
-  const hipError_t e = dense_lds(fn, p.wave_lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((run_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>), dim3(blocks), dim3(64),
-                     p.wave_lds, s, p);
-  return hipGetLastError();
+// split-mode (several problems per candidate) kernels exist for tile bounds up to 8
+template <class Sh> constexpr bool split_shape() { return Sh::TT <= 8; }
+
+template <class Sh, bool SPLIT> const void *run_dense_fn() {
+  return reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D, SPLIT>);
+}
+template <class Sh, bool SPLIT> const void *batch_dense_fn() {
+  return reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D, SPLIT>);
+}
+
+// F(integral_constant<bool, SPLIT>) for the requested form, hipErrorInvalidValue where the
+// shape has no split-mode kernel
+template <class Sh, class F> hipError_t with_split(bool split, F &&f) {
+  if (!split) return f(std::false_type{});
+  if constexpr (split_shape<Sh>()) return f(std::true_type{});
+  return hipErrorInvalidValue;
+}
+
+template <class Sh> hipError_t run_dense_k(const RunParams &p, uint32_t blocks, hipStream_t s,
+                                            bool split) {
+  return with_split<Sh>(split, [&](auto sp) {
+    constexpr bool SPLIT = decltype(sp)::value;
+    const hipError_t e = dense_lds(run_dense_fn<Sh, SPLIT>(), p.wave_lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((run_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D, SPLIT>), dim3(blocks),
+                       dim3(64), p.wave_lds, s, p);
+    return hipGetLastError();
+  });
 }
 
 template <class Sh> hipError_t batch_dense_k(const BatchParams &bp, uint32_t blocks,
-                                              hipStream_t s) {
-  hipLaunchKernelGGL((batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>), dim3(blocks), dim3(64),
-                     bp.wave_lds, s, bp);
-  return hipGetLastError();
+                                              hipStream_t s, bool split) {
+  return with_split<Sh>(split, [&](auto sp) {
+    constexpr bool SPLIT = decltype(sp)::value;
+    hipLaunchKernelGGL((batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D, SPLIT>), dim3(blocks),
+                       dim3(64), bp.wave_lds, s, bp);
+    return hipGetLastError();
+  });
 }
 
 // the paired form exists for the tile bounds up to 8 (256 sites)
@@ -943,41 +1036,53 @@ uint32_t dense_groups(uint32_t tile_bound) {
   return g;
 }
 
-uint32_t dense_waves_per_cu(uint32_t tile_bound, size_t wave_lds, bool persistent) {
+uint32_t dense_waves_per_cu(uint32_t tile_bound, size_t wave_lds, bool persistent,
+                            bool split) {
   int n = 0;
   const hipError_t e = with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    const void *fn = persistent
-                         ? reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>)
-                         : reinterpret_cast<const void *>(&run_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>);
-    hipError_t r = dense_lds(fn, wave_lds);
-    if (r != hipSuccess) return r;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, wave_lds);
+    return with_split<Sh>(split, [&](auto sp) {
+      constexpr bool SPLIT = decltype(sp)::value;
+      const void *fn = persistent ? batch_dense_fn<Sh, SPLIT>() : run_dense_fn<Sh, SPLIT>();
+      hipError_t r = dense_lds(fn, wave_lds);
+      if (r != hipSuccess) return r;
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, wave_lds);
+    });
   });
   return e == hipSuccess && n > 0 ? static_cast<uint32_t>(n) : 1u;
 }
 
+bool dense_split_available(uint32_t tile_bound) {
+  bool ok = false;
+  (void)with_dense_shape(tile_bound, [&](auto sh) {
+    ok = split_shape<decltype(sh)>();
+    return hipSuccess;
+  });
+  return ok;
+}
+
 hipError_t launch_run_dense(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
-                            hipStream_t s) {
+                            hipStream_t s, bool split) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return run_dense_k<Sh>(p, blocks, s);
+    return run_dense_k<Sh>(p, blocks, s, split);
   });
 }
 
-hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds) {
+hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds, bool split) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return dense_lds(
-        reinterpret_cast<const void *>(&batch_dense_kernel<Sh::TT, Sh::G, Sh::W, Sh::D>), lds);
+    return with_split<Sh>(split, [&](auto sp) {
+      return dense_lds(batch_dense_fn<Sh, decltype(sp)::value>(), lds);
+    });
   });
 }
 
 hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32_t blocks,
-                              hipStream_t s) {
+                              hipStream_t s, bool split) {
   return with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    return batch_dense_k<Sh>(bp, blocks, s);
+    return batch_dense_k<Sh>(bp, blocks, s, split);
   });
 }
 

@@ -18,6 +18,11 @@
 
 namespace sqsg {
 
+const char *const kDenseDomain =
+    "the dense kernel needs problems of at most 1024 sites, 16 species and 127 upper "
+    "neighbours per site and shell (split jobs: Monte Carlo, at most 256 sites per "
+    "sublattice)";
+
 void hip_check(hipError_t e, const char *what) {
   if (e == hipErrorOutOfMemory) throw std::bad_alloc();
   if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
@@ -226,10 +231,11 @@ void Problem::build_lane(uint32_t field_bits) {
 // order, zero tiles included (tiles past the problem's T tiles are zeros), so a tile's
 // offset is a function of the shell and the grid position alone and a prefetch of up to
 // one shell past the end stays inside the stream: dtiles_n = (S + 1) TT (TT + 1) / 2.
-void Problem::build_dense() {
+void Problem::build_dense(uint32_t TT) {
   if (d_dtiles.size()) return;
   dT = (N + 31) / 32;
-  const uint32_t n32 = 32 * dT, TT = dense_tile_bound(dT);
+  const uint32_t n32 = 32 * dT;
+  if (TT < dT) throw std::logic_error("tile bound below the problem's tile count");
   std::vector<uint8_t> sh(size_t(n32) * n32, 0xff);
   for (size_t q = 0; q < pi.size(); ++q) sh[size_t(pi[q]) * n32 + pj[q]] = static_cast<uint8_t>(ps[q]);
   dtiles_nonzero = 0;
@@ -298,13 +304,16 @@ void Problem::build_dense() {
 void ensure_dense(sqsg_job *job) {
   ensure_device(job);
   if (!job->dense_ok)
-    throw std::invalid_argument("the dense kernel needs one sublattice problem of at most 1024 "
-                                "sites, 16 species and 127 upper neighbours per site and shell");
-  for (auto &P : job->problems) P.build_dense();
+    throw std::invalid_argument(kDenseDomain);
+  // the problems of a split job share the kernel's tile bound
+  const uint32_t TT = dense_tile_bound_of(job);
+  for (auto &P : job->problems) P.build_dense(TT);
 }
 
 uint32_t dense_tile_bound_of(const sqsg_job *job) {
-  return dense_tile_bound((job->problems[0].N + 31) / 32);
+  uint32_t tiles = 0;
+  for (const auto &P : job->problems) tiles = std::max(tiles, (P.N + 31) / 32);
+  return dense_tile_bound(tiles);
 }
 
 // kernel applicability (host data only)
@@ -326,8 +335,14 @@ void analyze_job(sqsg_job *job) {
     for (size_t q = 0; q < P.pi.size(); ++q)
       P.max_upper = std::max(P.max_upper, ++deg[size_t(P.ps[q]) * P.N + P.pi[q]]);
   }
-  job->dense_ok = job->problems.size() == 1 && job->problems[0].N <= 1024 &&
-                  job->problems[0].K <= 16 && job->problems[0].max_upper <= 127;
+  // one problem of up to 1024 sites, or the sublattice problems of a split job (a form for
+  // Monte Carlo runs up to the tile bound dense_split_available reports: 256 sites each)
+  job->dense_ok = true;
+  for (const auto &P : job->problems)
+    job->dense_ok = job->dense_ok && P.N <= 1024 && P.K <= 16 && P.max_upper <= 127;
+  if (job->problems.size() > 1)
+    job->dense_ok = job->dense_ok && job->iteration_mode != SQSG_ITERATION_SYSTEMATIC &&
+                    dense_split_available(dense_tile_bound_of(job));
   job->analyzed = true;
 }
 
@@ -542,7 +557,8 @@ std::string kernel_shape(sqsg_job *job) {
 }
 
 bool paired_ok(const sqsg_job *job) {
-  return job->dense_ok && job->iteration_mode != SQSG_ITERATION_SYSTEMATIC &&
+  return job->dense_ok && job->problems.size() == 1 &&
+         job->iteration_mode != SQSG_ITERATION_SYSTEMATIC &&
          dense_paired_available(dense_tile_bound_of(job));
 }
 
@@ -560,10 +576,7 @@ int resolve_kernel(const sqsg_job *job, int requested) {
                                     "65536 pairs per shell");
       return requested;
     case SQSG_KERNEL_DENSE_MFMA:
-      if (!job->dense_ok)
-        throw std::invalid_argument("the dense kernel needs one sublattice problem of at most "
-                                    "1024 sites, 16 species and 127 upper neighbours per site "
-                                    "and shell");
+      if (!job->dense_ok) throw std::invalid_argument(kDenseDomain);
       return requested;
     case SQSG_KERNEL_DENSE_PAIRED:
       if (!paired_ok(job))
@@ -642,28 +655,43 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     return p;
   }
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
-    // rows of gen_group candidates (npass passes of dense_cw), then one shell's counts
-    const Problem &P = job->problems[0];
-    const uint32_t T = (P.N + 31) / 32, G = dense_groups(dense_tile_bound(T));
-    const uint32_t g = std::min(32 / P.K, 64 / G);
+    // per problem rows of gen_group candidates (npass passes of dense_cw), then one shell's
+    // counts; split jobs: the problems share the tile bound, the row stride and the pass
+    // width of the largest species count, and each lane's objectives per problem follow
+    const uint32_t nprob = p.nprob;
+    uint32_t T = 0, Kmax = 0;
+    for (const auto &P : job->problems) {
+      T = std::max(T, (P.N + 31) / 32);
+      Kmax = std::max(Kmax, P.K);
+    }
+    const uint32_t TT = dense_tile_bound(T), G = dense_groups(TT);
+    const uint32_t g = std::min(32 / Kmax, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
     // one shell's counts of a pass and 32 spare words; then the tie filter levels
-    const uint32_t cnt_bytes = align(p.dense_cw * P.K * P.K * 4 + 32 * 4);
+    const uint32_t cnt_bytes = align(p.dense_cw * Kmax * Kmax * 4 + 32 * 4);
     const uint32_t lf_bytes = align(static_cast<uint32_t>(sizeof(LevelFilter)));
+    const uint32_t obj_bytes = nprob > 1 ? align(nprob * 64 * 8) : 0;
+    auto rows_bytes = [&](uint32_t npass) {
+      const uint32_t r = npass * p.dense_cw * p.dense_stride;
+      return nprob == 1 ? r : nprob * align(r);
+    };
     uint32_t npass = 64 / p.dense_cw;
     // at most 20 KiB per wavefront: eight resident wavefronts per CU (two per SIMD)
-    while (npass > 1 && npass * p.dense_cw * p.dense_stride + cnt_bytes + lf_bytes > 20 * 1024)
+    while (npass > 1 && rows_bytes(npass) + cnt_bytes + obj_bytes + lf_bytes > 20 * 1024)
       --npass;
     p.gen_group = npass * p.dense_cw;
-    p.off_rows[0] = 0;
-    p.off_hist = align(p.gen_group * p.dense_stride);
-    p.off_levels = p.off_hist + cnt_bytes;
+    const uint32_t rb = align(p.gen_group * p.dense_stride);
+    for (uint32_t q = 0; q < nprob; ++q) p.off_rows[q] = q * rb;
+    p.off_hist = nprob * rb;
+    p.off_obj = p.off_hist + cnt_bytes;
+    p.off_levels = p.off_obj + obj_bytes;
     p.wave_lds = p.off_levels + lf_bytes;
     if (p.wave_lds > 160 * 1024) throw std::out_of_range("problem does not fit the LDS budget");
+    p.prune = job->prune && nprob == 1 ? 1u : 0u;  // the early exit bounds one problem's terms
     // one wavefront per block, as many as registers and LDS keep resident
     *blocks = static_cast<uint32_t>(device_cu_count()) *
-              dense_waves_per_cu(dense_tile_bound(T), p.wave_lds, false);
+              dense_waves_per_cu(TT, p.wave_lds, false, nprob > 1);
     return p;
   }
   if (variant == SQSG_KERNEL_SPARSE_LANE) {
@@ -702,7 +730,7 @@ hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, 
                           uint32_t W, hipStream_t s) {
   if (variant == SQSG_KERNEL_SPARSE_LANE) return launch_run_lane(p, job->kmax, blocks, W, s);
   if (variant == SQSG_KERNEL_DENSE_MFMA)
-    return launch_run_dense(p, dense_tile_bound_of(job), blocks, s);
+    return launch_run_dense(p, dense_tile_bound_of(job), blocks, s, job->problems.size() > 1);
   if (variant == SQSG_KERNEL_DENSE_PAIRED)
     return launch_run_dense_paired(p, dense_tile_bound_of(job), blocks, s);
   return launch_run_wave(p, blocks, W, s);

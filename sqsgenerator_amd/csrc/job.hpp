@@ -94,7 +94,7 @@ struct Problem {
   // of the bond count c = c_ab + c_ba; per shell its first value and its term stride
   size_t otab_n = 0;
   DevBuf d_otab, d_otab_shell;
-  void build_dense();
+  void build_dense(uint32_t tile_bound);  // the kernel's tile bound (the job's largest)
   DevProblem dev() const;
   void upload_device();
   uint32_t row_stride() const { return (((N + 3) / 4) | 1u) * 4u; }

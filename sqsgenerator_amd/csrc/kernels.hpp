@@ -35,13 +35,17 @@ uint32_t lane_field_bits(uint32_t kmax);
 // dense contraction on the MFMA cores; tile_bound = dense_tile_bound(32-site tiles)
 uint32_t dense_tile_bound(uint32_t tiles);
 uint32_t dense_groups(uint32_t tile_bound);  // 32-column tile groups per wavefront
+// split: the form for split-mode jobs (several sublattice problems per candidate), which
+// exists for the tile bounds dense_split_available reports
+bool dense_split_available(uint32_t tile_bound);
 // resident 1-wave blocks per CU of the dense kernel (registers and LDS), at least 1
-uint32_t dense_waves_per_cu(uint32_t tile_bound, size_t wave_lds, bool persistent);
+uint32_t dense_waves_per_cu(uint32_t tile_bound, size_t wave_lds, bool persistent,
+                            bool split = false);
 hipError_t launch_run_dense(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
-                            hipStream_t stream);
+                            hipStream_t stream, bool split = false);
 hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32_t blocks,
-                              hipStream_t stream);
-hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds);
+                              hipStream_t stream, bool split = false);
+hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds, bool split = false);
 hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStream_t stream);
 // paired form (MFMA and helper wavefronts, four pairs per block of 512 threads; MC jobs of
 // tile bounds whose shape runs two wavefronts per SIMD)

@@ -39,7 +39,8 @@ WORKLOADS = {
 
 
 KERNELS = {"wave": 1, "lane": 2, "dense": 3}
-SINGLE_PROBLEM = [n for n in WORKLOADS if not n.endswith("_split")]  # dense kernel domain
+SINGLE_PROBLEM = [n for n in WORKLOADS if not n.endswith("_split")]
+DENSE = list(WORKLOADS)  # dense kernel domain: split jobs too (sublattices up to 256 sites)
 PAIRED = ["fcc32_3shells", "fcc108_ternary", "bcc128_quaternary", "hcp128_binary",
           "fcc256_quinary"]  # paired dense form: single problem, at most 256 sites
 
@@ -48,8 +49,6 @@ PAIRED = ["fcc32_3shells", "fcc108_ternary", "bcc128_quaternary", "hcp128_binary
 @pytest.mark.parametrize("name", list(WORKLOADS))
 def test_score_batch_bit_exact(gpu, name, kernel):
     print(f"[score_batch] {name} {kernel}", flush=True)
-    if kernel == "dense" and name not in SINGLE_PROBLEM:
-        pytest.skip("the dense kernel scores jobs of one sublattice problem")
     cfg = parse_config(WORKLOADS[name]())
     job = Job(cfg)
     rng = np.random.default_rng(20261203)
@@ -190,7 +189,7 @@ def test_variants_agree_mc(gpu, name):
     d["iterations"] = 200_000 if name != "fcc864_quinary" else 20_000
     d["chunk_size"] = min(d["iterations"], 50_000)
     packs = []
-    kernels = (1, 2, 3, 4) if name in PAIRED else (1, 2, 3) if name in SINGLE_PROBLEM else (1, 2)
+    kernels = (1, 2, 3, 4) if name in PAIRED else (1, 2, 3)
     for kernel in kernels:
         job = Job(parse_config(d))
         job.set_kernel(kernel)

@@ -60,13 +60,26 @@ def test_systematic_pack_matches_oracle(gpu, kernel, keep, max_results, chunk):
     job.close()
 
 
+def _split_uneven(seed=14):
+    """rocksalt 3x3x3 in split mode with sublattices of different size and species count:
+    the 108 cation sites hold Ti/Al/Zr, 54 of the anion sites N/B, the others stay N"""
+    d = W.rocksalt_split_216(seed=seed)
+    st = parse_config(d).structure()
+    anions = [i for i, z in enumerate(st.species) if z == 7][:54]
+    d["composition"] = [{"sites": "Ti", "Ti": 36, "Al": 36, "Zr": 36},
+                        {"sites": anions, "N": 27, "B": 27}]
+    d["target_objective"] = [0, 0]
+    return d
+
+
 MC_CASES = {
     # name: (config, iterations, chunk, kernels)
     "sys16_mc": (lambda: dict(_sys16(), iteration_mode="random", seed=5), 30001, 7000,
                  (1, 2, 3, 4)),
     "fcc32": (lambda: W.plain_mc_32(seed=11), 50001, 10000, (1, 2, 3, 4)),
     "fcc256": (lambda: W.quinary_256(seed=12), 20001, 6000, (2, 3)),
-    "rocksalt_split": (lambda: W.rocksalt_split_216(seed=13), 5001, 1500, (1, 2)),
+    "rocksalt_split": (lambda: W.rocksalt_split_216(seed=13), 5001, 1500, (1, 2, 3)),
+    "split_uneven": (_split_uneven, 5001, 1500, (1, 2, 3)),
 }
 
 
