@@ -560,6 +560,33 @@ __global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
   if (last_job != 0xffffffffu && lane == 0 && scored) atomicAdd(bp.jobs[last_job].scored, scored);
 }
 
+// LDS rows of explicit candidates [b0, b0 + n) of a B x N batch (row stride `stride`,
+// 32 T bytes each, sites at or past N set to the padding byte 0xff); candidates at or past B
+// get species 0 (scored, never reported).  The wavefront copies the rows together: with
+// N a multiple of 16, 16-byte words of consecutive lanes read consecutive batch bytes.
+__device__ __forceinline__ void stage_rows(uint8_t *rows, uint32_t stride, const uint8_t *species,
+                                           uint64_t b0, uint32_t n, uint64_t B, uint32_t N,
+                                           uint32_t T, int lane) {
+  if ((N & 15u) == 0) {
+    const uint32_t per_row = 2 * T, chunks = n * per_row;  // 16-byte words
+#pragma unroll 4
+    for (uint32_t w = static_cast<uint32_t>(lane); w < chunks; w += 64) {
+      const uint32_t c = w / per_row, k = w - c * per_row;
+      uint4 v = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+      if (16 * k < N)
+        v = b0 + c < B ? *reinterpret_cast<const uint4 *>(species + (b0 + c) * N + 16 * k)
+                       : uint4{0, 0, 0, 0};
+      *reinterpret_cast<uint4 *>(rows + c * stride + 16 * k) = v;
+    }
+    return;
+  }
+  if (static_cast<uint32_t>(lane) >= n) return;  // one lane per row, byte by byte
+  const uint64_t b = b0 + static_cast<uint32_t>(lane);
+  uint8_t *row = rows + lane * stride;
+  for (uint32_t i = 0; i < 32 * T; ++i)
+    row[i] = (b < B && i < N) ? species[b * N + i] : (i < N ? 0 : 0xff);
+}
+
 template <int TT, int G, int D>
 __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -572,19 +599,8 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * GEN + lane;
   const bool gl = static_cast<uint32_t>(lane) < GEN;
   const bool active = gl && b < p.B;
-  if (gl) {
-    uint8_t *row = rows + lane * p.dense_stride;
-    if ((pr.N & 3u) == 0) {  // whole words (rows of the batch stay 4-byte aligned)
-      const uint32_t *src =
-          reinterpret_cast<const uint32_t *>(p.species + (active ? b : 0) * pr.N);
-      uint32_t *dst = reinterpret_cast<uint32_t *>(row);
-      for (uint32_t w = 0; w < 8 * pr.dT; ++w)
-        dst[w] = 4 * w < pr.N ? (active ? src[w] : 0u) : 0xffffffffu;
-    } else {
-      for (uint32_t i = 0; i < 32 * pr.dT; ++i)
-        row[i] = (active && i < pr.N) ? p.species[b * pr.N + i] : (i < pr.N ? 0 : 0xff);
-    }
-  }
+  stage_rows(rows, p.dense_stride, p.species, static_cast<uint64_t>(blockIdx.x) * GEN, GEN, p.B,
+             pr.N, pr.dT, lane);
   const unsigned long long am = __ballot(active);
   unsigned long long live = 0;
   for (uint32_t q = 0; q < npass; ++q)
@@ -596,6 +612,180 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
       pr, rows, p.dense_stride, cnt, CW, npass, live, lane,
       p.bonds ? p.bonds + b0 * nb : nullptr, p.sro ? p.sro + b0 * nb : nullptr, out_rows);
   if (active) p.objective[b] = obj;
+}
+
+// ---- explicit scoring split by shell -----------------------------------------------------
+//
+// For batches too small to fill the device with whole candidates: a block of kShellWaves
+// wavefronts scores one shell s (blockIdx.y) for kShellWaves passes of CW candidates (one
+// pass per wavefront).  The shell's tile grid (tile bounds up to 8: at most 36 KiB) is loaded
+// into LDS once per block and shared by its wavefronts, which then run all their grid
+// positions without further barriers.  Each candidate's lane evaluates the shell's terms
+// from the counts: the values go to `vals` [T][B] (term-major, so the final sum reads them
+// coalesced), the SRO and bond entries to the outputs.  score_sum_kernel then adds each
+// candidate's term values from 0.0 in the reference order (optimization.h:85-100): the same
+// values in the same order as the whole-candidate kernels, so objectives are bit-identical.
+constexpr uint32_t kShellWaves = 4;
+
+// The values of one shell's terms of a candidate (its counts at cc): term t of shell s goes
+// to vals[(t0 + t) * B], t0 = s K (K + 1) / 2; bonds and SRO entries when given.  With K
+// known at compile time the term parameters come in one batch of scalar loads.
+template <int KT, class PR>
+__device__ __forceinline__ void shell_vals_k(const PR &pr, uint32_t s, const uint32_t *cc,
+                                             double *vals, uint64_t B, int32_t *bonds,
+                                             double *sro, uint32_t Kr) {
+  const uint32_t K = KT > 0 ? static_cast<uint32_t>(KT) : Kr;
+  const uint32_t t0 = s * (K * (K + 1) / 2);
+  const TermParam *tp = pr.tparams + t0;
+  uint32_t t = 0;
+#pragma unroll
+  for (uint32_t xi = 0; xi < K; ++xi)
+#pragma unroll
+    for (uint32_t eta = xi; eta < K; ++eta, ++t) {
+      const auto &e = constant_ref(tp, t);
+      const uint32_t cab = cc[xi * K + eta];
+      const uint32_t cba = xi == eta ? 0u : cc[eta * K + xi];
+      const double v = 1.0 - static_cast<double>(cab + cba) * e.pref;
+      vals[static_cast<uint64_t>(t0 + t) * B] = e.pw * absolute(v - e.tgt);
+      if (sro) sro[(s * K + xi) * K + eta] = v, sro[(s * K + eta) * K + xi] = v;
+      if (bonds) {
+        bonds[(s * K + xi) * K + eta] = static_cast<int32_t>(cab);
+        bonds[(s * K + eta) * K + xi] = static_cast<int32_t>(cc[eta * K + xi]);
+      }
+    }
+}
+
+template <class PR>
+__device__ __forceinline__ void shell_vals(const PR &pr, uint32_t s, const uint32_t *cc,
+                                           double *vals, uint64_t B, int32_t *bonds,
+                                           double *sro) {
+  switch (pr.K) {
+    case 2: return shell_vals_k<2>(pr, s, cc, vals, B, bonds, sro, 2);
+    case 3: return shell_vals_k<3>(pr, s, cc, vals, B, bonds, sro, 3);
+    case 4: return shell_vals_k<4>(pr, s, cc, vals, B, bonds, sro, 4);
+    case 5: return shell_vals_k<5>(pr, s, cc, vals, B, bonds, sro, 5);
+    case 6: return shell_vals_k<6>(pr, s, cc, vals, B, bonds, sro, 6);
+    default: return shell_vals_k<0>(pr, s, cc, vals, B, bonds, sro, pr.K);
+  }
+}
+
+template <int TT, int G>
+__global__ __launch_bounds__(64 * kShellWaves, 2) void score_shell_kernel(ScoreParams p,
+                                                                          double *vals) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  constexpr int W = static_cast<int>(kShellWaves);
+  constexpr int P = TT * (TT + 1) / 2;
+  const uint32_t wave = threadIdx.x >> 6;
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  const DevProblem &pr = p.prob;
+  const uint32_t K = pr.K, KK = K * K, S = pr.S, T = pr.dT, s = blockIdx.y;
+  const uint32_t CW = p.dense_cw, g = CW / G, stride = p.dense_stride;
+  v4i *tbuf = reinterpret_cast<v4i *>(lds);  // the shell's P tiles (64 chunks each)
+  uint8_t *rows = lds + P * 1024 + wave * p.shell_rows;
+  // CW x K x K counts, over the rows: the wavefront writes them after its one-hot tiles are
+  // built (LDS operations of one wavefront stay in program order)
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(rows);
+  // this wavefront's pass: candidates [pass * CW, pass * CW + CW), rows staged in LDS
+  const uint64_t pass = static_cast<uint64_t>(blockIdx.x) * W + wave;
+  const uint64_t b = pass * CW + static_cast<uint32_t>(lane);
+  const bool gl = static_cast<uint32_t>(lane) < CW;
+  const bool active = gl && b < p.B;
+  stage_rows(rows, stride, p.species, pass * CW, CW, p.B, pr.N, T, lane);
+  // the shell's tiles: every load of a thread is issued before the first store
+  {
+    const v4i *stream = reinterpret_cast<const v4i *>(pr.dtiles) + tile_offset<TT>(s, 0) / 16;
+    constexpr int NCH = (P * 64 + 64 * W - 1) / (64 * W);
+    v4i t[NCH];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+      const int idx = static_cast<int>(threadIdx.x) + k * 64 * W;
+      if (idx < P * 64) t[k] = stream[idx];
+    }
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+      const int idx = static_cast<int>(threadIdx.x) + k * 64 * W;
+      if (idx < P * 64) tbuf[idx] = t[k];
+    }
+  }
+  __syncthreads();  // tiles and rows in LDS
+  const uint32_t h = static_cast<uint32_t>(lane) >> 5, n = static_cast<uint32_t>(lane) & 31u;
+  const uint32_t cl = n / K, bcol = n - cl * K;
+  const bool col_ok = cl < g;
+  const uint32_t bb = bcol * 0x01010101u;
+  v4i X[G][TT];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    const uint32_t c = gi * g + cl;
+    const uint8_t *row = rows + (col_ok ? c : 0u) * stride + 16 * h;
+#pragma unroll
+    for (int J = 0; J < TT; ++J) {
+      v4i x = {0, 0, 0, 0};
+      if (static_cast<uint32_t>(J) < T && col_ok) {
+        const uint4 w = *reinterpret_cast<const uint4 *>(row + 32 * J);
+        x[0] = onehot4(w.x, bb);
+        x[1] = onehot4(w.y, bb);
+        x[2] = onehot4(w.z, bb);
+        x[3] = onehot4(w.w, bb);
+      }
+      X[gi][J] = x;
+    }
+  }
+  v16i C[G], Y[G];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) C[gi] = v16i{};
+  const v4i *tl = tbuf + lane;
+  int q = 0;  // grid position, row-major over I <= J < TT
+#pragma unroll
+  for (int I = 0; I < TT; ++I) {
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
+#pragma unroll
+    for (int J = I; J < TT; ++J, ++q) {
+      const v4i u = tl[q * 64];
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi)
+        Y[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(u, X[gi][J], Y[gi], 0, 0, 0);
+    }
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+      C[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(X[gi][I], pack_counts(Y[gi]), C[gi], 0, 0,
+                                                    0);
+  }
+  // diagonal blocks -> counts [candidate of the pass][a][b]: register r of lane l is row
+  // m = 8(r/4) + 4h + r%4 = cK + a of column n = cK + b
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+    for (uint32_t r = 0; r < 16; ++r) {
+      const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
+      if (col_ok && m / K == cl)  // species a = m - cK of the candidate's row
+        cnt[(gi * g + cl) * KK + (m - cl * K) * K + bcol] = static_cast<uint32_t>(C[gi][r]);
+    }
+  wave_sync();
+  // the shell's terms of the lane's candidate, in the reference order (xi, then eta >= xi)
+  if (active) {
+    const uint32_t nb = S * KK;
+    shell_vals(pr, s, cnt + static_cast<uint32_t>(lane) * KK, vals + b, p.B,
+               p.bonds ? p.bonds + b * nb : nullptr, p.sro ? p.sro + b * nb : nullptr);
+  }
+}
+
+// objective of candidate b: its T term values added to 0.0 in the reference order
+__global__ __launch_bounds__(256) void score_sum_kernel(ScoreParams p, const double *vals) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= p.B) return;
+  const uint32_t T = p.prob.T;
+  double obj = 0.0;
+  for (uint32_t t0 = 0; t0 < T; t0 += 64) {  // 64 loads in flight, then 64 ordered adds
+    double v[64];
+#pragma unroll
+    for (uint32_t j = 0; j < 64; ++j)
+      if (t0 + j < T) v[j] = vals[static_cast<uint64_t>(t0 + j) * p.B + b];
+#pragma unroll
+    for (uint32_t j = 0; j < 64; ++j)
+      if (t0 + j < T) obj += v[j];
+  }
+  p.objective[b] = obj;
 }
 
 // ---- paired form (MC jobs) -------------------------------------------------------------
@@ -1020,7 +1210,50 @@ template <class Sh> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s
   return score_dense_g<Sh::TT, Sh::G, Sh::DS>(p, s);
 }
 
+// the shell-split form exists for tile bounds up to 8 (a shell's tiles fit in LDS)
+template <class Sh> constexpr bool shell_shape() { return Sh::TT <= 8; }
+
+template <class Sh> hipError_t score_shell_k(const ScoreParams &p, double *vals, hipStream_t s) {
+  if constexpr (!shell_shape<Sh>()) {
+    return hipErrorInvalidValue;
+  } else {
+    const void *fn = reinterpret_cast<const void *>(&score_shell_kernel<Sh::TT, Sh::G>);
+    const hipError_t e = dense_lds(fn, p.wave_lds);
+    if (e != hipSuccess) return e;
+    const uint64_t passes = (p.B + p.dense_cw - 1) / p.dense_cw;
+    const dim3 grid(static_cast<uint32_t>((passes + kShellWaves - 1) / kShellWaves), p.prob.S);
+    hipLaunchKernelGGL((score_shell_kernel<Sh::TT, Sh::G>), grid, dim3(64 * kShellWaves),
+                       p.wave_lds, s, p, vals);
+    const hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(score_sum_kernel, dim3(static_cast<uint32_t>((p.B + 255) / 256)),
+                       dim3(256), 0, s, p, vals);
+    return hipGetLastError();
+  }
+}
+
 }  // namespace
+
+uint32_t dense_shell_waves() { return kShellWaves; }
+
+bool dense_shell_available(uint32_t tile_bound) {
+  bool ok = false;
+  (void)with_dense_shape(tile_bound, [&](auto sh) {
+    ok = shell_shape<decltype(sh)>();
+    return hipSuccess;
+  });
+  return ok;
+}
+
+hipError_t launch_score_dense_shells(const ScoreParams &p, uint32_t tile_bound, double *vals,
+                                     hipStream_t s) {
+  if (p.B == 0) return hipSuccess;
+  return with_dense_shape(tile_bound, [&](auto sh) {
+    using Sh = decltype(sh);
+    if (p.dense_groups != static_cast<uint32_t>(Sh::G)) return hipErrorInvalidValue;
+    return score_shell_k<Sh>(p, vals, s);
+  });
+}
 
 uint32_t dense_tile_bound(uint32_t tiles) {
   for (uint32_t tt : {1u, 2u, 4u, 6u, 8u, 12u, 14u, 16u, 20u, 24u, 27u, 32u})

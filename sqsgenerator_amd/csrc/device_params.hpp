@@ -149,6 +149,7 @@ struct ScoreParams {
   uint32_t wave_lds, off_row, off_hist, off_terms;
   uint32_t dense_cw, dense_gen, dense_stride;
   uint32_t dense_groups;       // tile groups per wavefront of the dense kernel: 1 or the shape's
+  uint32_t shell_rows;         // shell-split kernel: LDS bytes of one wavefront's candidate rows
 };
 
 }  // namespace sqsg

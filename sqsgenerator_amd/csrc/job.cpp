@@ -996,12 +996,28 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     sp.prob = P.dev();
     sp.dense_cw = rp.dense_cw;
     sp.dense_groups = dense_groups(dense_tile_bound_of(job));
+    const uint64_t want = uint64_t(8) * static_cast<uint64_t>(device_cu_count());
+    const uint32_t TT = dense_tile_bound_of(job);
+    if (dense_shell_available(TT) && (uint64_t(B) + rp.dense_cw - 1) / rp.dense_cw < want) {
+      // a batch too small for a pass of whole candidates per wavefront: one wavefront per
+      // (pass, shell), the blocks sharing each shell's tiles in LDS, the term values in
+      // scratch, then their sums
+      sp.dense_stride = rp.dense_stride;
+      // a wavefront's candidate rows, later its counts (which need less)
+      sp.shell_rows = (std::max(rp.dense_cw * rp.dense_stride, rp.dense_cw * P.K * P.K * 4) +
+                       15u) & ~15u;
+      sp.wave_lds = TT * (TT + 1) / 2 * 1024 + dense_shell_waves() * sp.shell_rows;
+      const size_t bytes = size_t(B) * P.terms.size() * sizeof(double);
+      if (job->score_scratch.size() < bytes) job->score_scratch.alloc(bytes);
+      SQSG_HIP(launch_score_dense_shells(sp, TT, job->score_scratch.as<double>(),
+                                         s ? s : job_stream(job)));
+      return;
+    }
     // candidates per wavefront: whole passes, fewer of them when the batch would otherwise
     // leave most SIMDs idle (aim: two wavefronts per SIMD); a batch that does not fill them
     // with one pass per wavefront runs one tile group per wavefront (twice the wavefronts,
     // each with half the MFMA chain)
     uint32_t gen = rp.gen_group;
-    const uint64_t want = uint64_t(8) * static_cast<uint64_t>(device_cu_count());
     while (gen > rp.dense_cw && (uint64_t(B) + gen - 1) / gen < want) gen -= rp.dense_cw;
     if (gen == rp.dense_cw && sp.dense_groups > 1 && (uint64_t(B) + gen - 1) / gen < want) {
       sp.dense_cw = rp.dense_cw / sp.dense_groups;

@@ -168,6 +168,7 @@ struct sqsg_job {
   int64_t max_results = -1;
   int kernel = SQSG_KERNEL_AUTO;
   bool prune = false;     // early exit of hopeless passes (dense kernel, pw >= 0)
+  sqsg::DevBuf score_scratch;  // term values of explicit scoring split by shell
   uint64_t launch_size = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -47,6 +47,13 @@ hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32
                               hipStream_t stream, bool split = false);
 hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds, bool split = false);
 hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStream_t stream);
+// explicit scoring split by shell (batches too small to fill the device; tile bounds with
+// dense_shell_available): blocks of dense_shell_waves() wavefronts, one pass of the shape's
+// tile groups each; `vals` is scratch of B x T doubles (the candidates' term values)
+uint32_t dense_shell_waves();
+bool dense_shell_available(uint32_t tile_bound);
+hipError_t launch_score_dense_shells(const ScoreParams &p, uint32_t tile_bound, double *vals,
+                                     hipStream_t stream);
 // paired form (MFMA and helper wavefronts, four pairs per block of 512 threads; MC jobs of
 // tile bounds whose shape runs two wavefronts per SIMD)
 constexpr uint32_t kDensePairs = 4, kDensePairHead = 128;  // pairs per block, LDS header
