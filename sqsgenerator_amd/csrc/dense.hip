@@ -618,96 +618,96 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
 //
 // For batches too small to fill the device with whole candidates: a block of kShellWaves
 // wavefronts scores one shell s (blockIdx.y) for kShellWaves passes of CW candidates (one
-// pass per wavefront).  The shell's tile grid (tile bounds up to 8: at most 36 KiB) is loaded
-// into LDS once per block and shared by its wavefronts, which then run all their grid
-// positions without further barriers.  Each candidate's lane evaluates the shell's terms
-// from the counts: the values go to `vals` [T][B] (term-major, so the final sum reads them
-// coalesced), the SRO and bond entries to the outputs.  score_sum_kernel then adds each
-// candidate's term values from 0.0 in the reference order (optimization.h:85-100): the same
+// pass per wavefront).  The block's wavefronts share the shell's tiles in LDS: LT (tile
+// bounds up to 8, a shell's tiles at most 36 KiB) copies them once; larger bounds stream them
+// through kShellBufs LDS buffers of kShellChunk grid positions: the loads of chunk k + 3
+// are issued as chunk k starts (two chunks in registers at a time) and chunk k + 2 is stored
+// as chunk k ends, then one barrier.  Read from LDS, a tile costs 1 KiB of LDS bandwidth per
+// MFMA group; read by each wavefront from the L2 stream it would cost the same in L1
+// bandwidth, which holds half of LDS's.  The tile, term-parameter and row loads of a block
+// are all issued before the first is waited for.  After the GEMMs, all 64 lanes of a
+// wavefront work on its pass's outputs: the term values (lane items (term, candidate),
+// candidate fastest) go to `vals` [T][B] (term-major: the sum kernel reads them coalesced),
+// the SRO entries and the bond counts to the outputs.  score_sum_kernel then adds each
+// candidate's T values from 0.0 in the reference order (optimization.h:85-100): the same
 // values in the same order as the whole-candidate kernels, so objectives are bit-identical.
+// Measured alternatives at 256 sites: folding the sum into the last of a column's S blocks
+// to finish needs a release fence at agent scope (an L2 write-back) in every block, 2.3x
+// slower; 16-bit bond counts instead of the values move the term arithmetic into the sum
+// kernel's serial per-candidate loop, 1.4x slower.
 constexpr uint32_t kShellWaves = 4;
+constexpr int kShellChunk = 16, kShellBufs = 3;
 
-// The values of one shell's terms of a candidate (its counts at cc): term t of shell s goes
-// to vals[(t0 + t) * B], t0 = s K (K + 1) / 2; bonds and SRO entries when given.  With K
-// known at compile time the term parameters come in one batch of scalar loads.
-template <int KT, class PR>
-__device__ __forceinline__ void shell_vals_k(const PR &pr, uint32_t s, const uint32_t *cc,
-                                             double *vals, uint64_t B, int32_t *bonds,
-                                             double *sro, uint32_t Kr) {
-  const uint32_t K = KT > 0 ? static_cast<uint32_t>(KT) : Kr;
-  const uint32_t t0 = s * (K * (K + 1) / 2);
-  const TermParam *tp = pr.tparams + t0;
-  uint32_t t = 0;
-#pragma unroll
-  for (uint32_t xi = 0; xi < K; ++xi)
-#pragma unroll
-    for (uint32_t eta = xi; eta < K; ++eta, ++t) {
-      const auto &e = constant_ref(tp, t);
-      const uint32_t cab = cc[xi * K + eta];
-      const uint32_t cba = xi == eta ? 0u : cc[eta * K + xi];
-      const double v = 1.0 - static_cast<double>(cab + cba) * e.pref;
-      vals[static_cast<uint64_t>(t0 + t) * B] = e.pw * absolute(v - e.tgt);
-      if (sro) sro[(s * K + xi) * K + eta] = v, sro[(s * K + eta) * K + xi] = v;
-      if (bonds) {
-        bonds[(s * K + xi) * K + eta] = static_cast<int32_t>(cab);
-        bonds[(s * K + eta) * K + xi] = static_cast<int32_t>(cc[eta * K + xi]);
-      }
-    }
-}
-
-template <class PR>
-__device__ __forceinline__ void shell_vals(const PR &pr, uint32_t s, const uint32_t *cc,
-                                           double *vals, uint64_t B, int32_t *bonds,
-                                           double *sro) {
-  switch (pr.K) {
-    case 2: return shell_vals_k<2>(pr, s, cc, vals, B, bonds, sro, 2);
-    case 3: return shell_vals_k<3>(pr, s, cc, vals, B, bonds, sro, 3);
-    case 4: return shell_vals_k<4>(pr, s, cc, vals, B, bonds, sro, 4);
-    case 5: return shell_vals_k<5>(pr, s, cc, vals, B, bonds, sro, 5);
-    case 6: return shell_vals_k<6>(pr, s, cc, vals, B, bonds, sro, 6);
-    default: return shell_vals_k<0>(pr, s, cc, vals, B, bonds, sro, pr.K);
-  }
-}
-
-template <int TT, int G>
-__global__ __launch_bounds__(64 * kShellWaves, 2) void score_shell_kernel(ScoreParams p,
-                                                                          double *vals) {
+template <int TT, int G, int D, bool LT>
+__global__ __launch_bounds__(64 * kShellWaves, LT ? 2 : 1) void score_shell_kernel(ScoreParams p,
+                                                                                   double *vals) {
   extern __shared__ __align__(16) uint8_t lds[];
   constexpr int W = static_cast<int>(kShellWaves);
   constexpr int P = TT * (TT + 1) / 2;
+  constexpr int Q = kShellChunk, NCK = (P + Q - 1) / Q;  // streamed form: chunks of Q tiles
+  constexpr int PER = Q * 64 / (64 * W);  // 16-byte loads of a chunk per thread
+  static_assert(LT || (Q * 64) % (64 * W) == 0, "a chunk splits evenly over the block");
+  constexpr uint32_t kTileLds = static_cast<uint32_t>(LT ? P : kShellBufs * Q) * 1024u;
   const uint32_t wave = threadIdx.x >> 6;
   const int lane = static_cast<int>(threadIdx.x & 63u);
   const DevProblem &pr = p.prob;
   const uint32_t K = pr.K, KK = K * K, S = pr.S, T = pr.dT, s = blockIdx.y;
+  const uint32_t NT = K * (K + 1) / 2;
   const uint32_t CW = p.dense_cw, g = CW / G, stride = p.dense_stride;
-  v4i *tbuf = reinterpret_cast<v4i *>(lds);  // the shell's P tiles (64 chunks each)
-  uint8_t *rows = lds + P * 1024 + wave * p.shell_rows;
-  // CW x K x K counts, over the rows: the wavefront writes them after its one-hot tiles are
-  // built (LDS operations of one wavefront stay in program order)
+  // LT: the shell's P tiles (64 16-byte words each); else kShellBufs chunks of Q tiles
+  v4i *tbuf = reinterpret_cast<v4i *>(lds);
+  TermParam *tpar = reinterpret_cast<TermParam *>(lds + kTileLds);  // the shell's NT terms
+  // the wavefront's candidate rows; after its one-hot tiles are built, its CW x K x K counts
+  // (LDS operations of one wavefront stay in program order)
+  uint8_t *rows = lds + p.off_row + wave * p.shell_rows;
   uint32_t *cnt = reinterpret_cast<uint32_t *>(rows);
-  // this wavefront's pass: candidates [pass * CW, pass * CW + CW), rows staged in LDS
+  // this wavefront's pass: candidates [b0, b0 + CW)
   const uint64_t pass = static_cast<uint64_t>(blockIdx.x) * W + wave;
-  const uint64_t b = pass * CW + static_cast<uint32_t>(lane);
-  const bool gl = static_cast<uint32_t>(lane) < CW;
-  const bool active = gl && b < p.B;
-  stage_rows(rows, stride, p.species, pass * CW, CW, p.B, pr.N, T, lane);
-  // the shell's tiles: every load of a thread is issued before the first store
-  {
-    const v4i *stream = reinterpret_cast<const v4i *>(pr.dtiles) + tile_offset<TT>(s, 0) / 16;
-    constexpr int NCH = (P * 64 + 64 * W - 1) / (64 * W);
-    v4i t[NCH];
+  const uint64_t b0 = pass * CW;
+  // the shell's tiles in the stream.  Streamed chunks may run past the shell's last
+  // position into the next shell (the stream ends with a zero shell), never past the stream.
+  const v4i *stream = reinterpret_cast<const v4i *>(pr.dtiles) + tile_offset<TT>(s, 0) / 16;
+  const int tid = static_cast<int>(threadIdx.x);
+  // loads in flight: the shell's tiles (LT) or its first two chunks and the third in its
+  // register set (chunk c in pf[c & 1]), the term parameters, then the rows
+  constexpr int NCH = LT ? (P * 64 + 64 * W - 1) / (64 * W) : 2 * PER;
+  v4i tl_in[NCH];
+  v4i pf[LT ? 1 : 2][PER];
+  if constexpr (LT) {
 #pragma unroll
     for (int k = 0; k < NCH; ++k) {
-      const int idx = static_cast<int>(threadIdx.x) + k * 64 * W;
-      if (idx < P * 64) t[k] = stream[idx];
+      const int idx = tid + k * 64 * W;
+      if (idx < P * 64) tl_in[k] = stream[idx];
     }
+  } else {
 #pragma unroll
-    for (int k = 0; k < NCH; ++k) {
-      const int idx = static_cast<int>(threadIdx.x) + k * 64 * W;
-      if (idx < P * 64) tbuf[idx] = t[k];
+    for (int k = 0; k < NCH; ++k)
+      if (k < PER || NCK > 1) tl_in[k] = stream[tid + k * 64 * W];
+    if (NCK > 2) {
+#pragma unroll
+      for (int j = 0; j < PER; ++j) pf[0][j] = stream[2 * Q * 64 + tid + j * 64 * W];
     }
   }
-  __syncthreads();  // tiles and rows in LDS
+  const uint32_t nprm = NT * (sizeof(TermParam) / 16);  // 16-byte words of the parameters
+  const uint4 *prm_src = reinterpret_cast<const uint4 *>(pr.tparams + s * NT);
+  uint4 prm = {0, 0, 0, 0};
+  if (threadIdx.x < nprm) prm = prm_src[threadIdx.x];
+  stage_rows(rows, stride, p.species, b0, CW, p.B, pr.N, T, lane);
+  if (threadIdx.x < nprm) reinterpret_cast<uint4 *>(tpar)[threadIdx.x] = prm;
+  for (uint32_t i = threadIdx.x + 64 * W; i < nprm; i += 64 * W)  // K > 8 only
+    reinterpret_cast<uint4 *>(tpar)[i] = prm_src[i];
+  if constexpr (LT) {
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+      const int idx = tid + k * 64 * W;
+      if (idx < P * 64) tbuf[idx] = tl_in[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NCH; ++k)
+      if (k < PER || NCK > 1) tbuf[tid + k * 64 * W] = tl_in[k];
+  }
+  __syncthreads();  // tiles, parameters and rows in LDS
   const uint32_t h = static_cast<uint32_t>(lane) >> 5, n = static_cast<uint32_t>(lane) & 31u;
   const uint32_t cl = n / K, bcol = n - cl * K;
   const bool col_ok = cl < g;
@@ -734,58 +734,110 @@ __global__ __launch_bounds__(64 * kShellWaves, 2) void score_shell_kernel(ScoreP
 #pragma unroll
   for (int gi = 0; gi < G; ++gi) C[gi] = v16i{};
   const v4i *tl = tbuf + lane;
-  int q = 0;  // grid position, row-major over I <= J < TT
+  // grid positions q row-major over I <= J < TT, stepped so that indexes fold to constants
+  int I = 0, J = 0;
 #pragma unroll
-  for (int I = 0; I < TT; ++I) {
+  for (int q = 0; q < P; ++q) {
+    if (J == I) {
 #pragma unroll
-    for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
-#pragma unroll
-    for (int J = I; J < TT; ++J, ++q) {
-      const v4i u = tl[q * 64];
-#pragma unroll
-      for (int gi = 0; gi < G; ++gi)
-        Y[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(u, X[gi][J], Y[gi], 0, 0, 0);
+      for (int gi = 0; gi < G; ++gi) Y[gi] = v16i{};
     }
+    const int k = q / Q, r = q % Q;  // streamed form: chunk k, position r in it
+    if (!LT && r == 0 && k + 3 < NCK) {
+#pragma unroll
+      for (int j = 0; j < PER; ++j)
+        pf[(k + 3) & 1][j] = stream[(k + 3) * Q * 64 + tid + j * 64 * W];
+    }
+    const v4i u = LT ? tl[q * 64] : tl[((k % kShellBufs) * Q + r) * 64];
 #pragma unroll
     for (int gi = 0; gi < G; ++gi)
-      C[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(X[gi][I], pack_counts(Y[gi]), C[gi], 0, 0,
-                                                    0);
+      Y[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(u, X[gi][J], Y[gi], 0, 0, 0);
+    if (J == TT - 1) {  // row I complete: C += X_I^T Y_I
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi)
+        C[gi] = __builtin_amdgcn_mfma_i32_32x32x32_i8(X[gi][I], pack_counts(Y[gi]), C[gi], 0,
+                                                      0, 0);
+    }
+    if (++J == TT) ++I, J = I;
+    // end of chunk k: chunk k + 2 into the buffer chunk k - 1 used (every wavefront is past
+    // it: the barrier at the end of chunk k - 1), then the barrier that publishes it
+    if (!LT && r == Q - 1 && k + 1 < NCK) {
+      if (k + 2 < NCK) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+          tbuf[((k + 2) % kShellBufs) * Q * 64 + tid + j * 64 * W] = pf[(k + 2) & 1][j];
+      }
+      __syncthreads();
+    }
   }
   // diagonal blocks -> counts [candidate of the pass][a][b]: register r of lane l is row
   // m = 8(r/4) + 4h + r%4 = cK + a of column n = cK + b
+  const uint32_t base = cl * K;
 #pragma unroll
   for (int gi = 0; gi < G; ++gi)
 #pragma unroll
     for (uint32_t r = 0; r < 16; ++r) {
-      const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
-      if (col_ok && m / K == cl)  // species a = m - cK of the candidate's row
-        cnt[(gi * g + cl) * KK + (m - cl * K) * K + bcol] = static_cast<uint32_t>(C[gi][r]);
+      const uint32_t a = 8 * (r >> 2) + 4 * h + (r & 3) - base;  // wraps below the block
+      if (col_ok && a < K) cnt[(gi * g + cl) * KK + a * K + bcol] = static_cast<uint32_t>(C[gi][r]);
     }
   wave_sync();
-  // the shell's terms of the lane's candidate, in the reference order (xi, then eta >= xi)
-  if (active) {
-    const uint32_t nb = S * KK;
-    shell_vals(pr, s, cnt + static_cast<uint32_t>(lane) * KK, vals + b, p.B,
-               p.bonds ? p.bonds + b * nb : nullptr, p.sro ? p.sro + b * nb : nullptr);
+  // the pass's term values of shell s, its SRO entries and bond counts
+  const uint32_t cube = S * KK, t0 = s * NT;
+  for (uint32_t i = static_cast<uint32_t>(lane); i < NT * CW; i += 64) {
+    const uint32_t t = i / CW, c = i - t * CW;
+    const uint64_t b = b0 + c;
+    if (b >= p.B) continue;
+    const TermParam &e = tpar[t];
+    const uint32_t *cc = cnt + c * KK;
+    const uint32_t cab = cc[e.xi * K + e.eta];
+    const uint32_t cba = e.xi == e.eta ? 0u : cc[e.eta * K + e.xi];
+    const double v = 1.0 - static_cast<double>(cab + cba) * e.pref;
+    vals[static_cast<uint64_t>(t0 + t) * p.B + b] = e.pw * absolute(v - e.tgt);
+    if (p.sro) {
+      double *o = p.sro + b * cube + s * KK;
+      o[e.xi * K + e.eta] = v;
+      o[e.eta * K + e.xi] = v;
+    }
+  }
+  if (p.bonds) {
+    for (uint32_t i = static_cast<uint32_t>(lane); i < CW * KK; i += 64) {
+      const uint32_t c = i / KK, k = i - c * KK;
+      const uint64_t b = b0 + c;
+      if (b < p.B) p.bonds[b * cube + s * KK + k] = static_cast<int32_t>(cnt[i]);
+    }
   }
 }
 
-// objective of candidate b: its T term values added to 0.0 in the reference order
-__global__ __launch_bounds__(256) void score_sum_kernel(ScoreParams p, const double *vals) {
-  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= p.B) return;
+// objectives of kSumCands candidates per block: their T term values, staged through LDS in
+// chunks of kSumTerms terms (loads spread over the block, coalesced along candidates), added
+// to 0.0 in the reference order by one thread per candidate
+constexpr uint32_t kSumCands = 16, kSumTerms = 256, kSumThreads = 256;
+
+__global__ __launch_bounds__(kSumThreads) void score_sum_kernel(ScoreParams p,
+                                                               const double *vals) {
+  __shared__ double v[kSumTerms * kSumCands];
+  const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kSumCands;
   const uint32_t T = p.prob.T;
   double obj = 0.0;
-  for (uint32_t t0 = 0; t0 < T; t0 += 64) {  // 64 loads in flight, then 64 ordered adds
-    double v[64];
+  for (uint32_t t0 = 0; t0 < T; t0 += kSumTerms) {
+    constexpr uint32_t kPer = kSumTerms * kSumCands / kSumThreads;
+    double x[kPer];
 #pragma unroll
-    for (uint32_t j = 0; j < 64; ++j)
-      if (t0 + j < T) v[j] = vals[static_cast<uint64_t>(t0 + j) * p.B + b];
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t i = threadIdx.x + k * kSumThreads;
+      const uint32_t t = t0 + i / kSumCands, c = i % kSumCands;
+      x[k] = t < T && b0 + c < p.B ? vals[static_cast<uint64_t>(t) * p.B + b0 + c] : 0.0;
+    }
 #pragma unroll
-    for (uint32_t j = 0; j < 64; ++j)
-      if (t0 + j < T) obj += v[j];
+    for (uint32_t k = 0; k < kPer; ++k) v[threadIdx.x + k * kSumThreads] = x[k];
+    __syncthreads();
+    if (threadIdx.x < kSumCands) {
+      const uint32_t nt = T - t0 < kSumTerms ? T - t0 : kSumTerms;
+      for (uint32_t t = 0; t < nt; ++t) obj += v[t * kSumCands + threadIdx.x];
+    }
+    __syncthreads();  // the next chunk overwrites the stage
   }
-  p.objective[b] = obj;
+  if (threadIdx.x < kSumCands && b0 + threadIdx.x < p.B) p.objective[b0 + threadIdx.x] = obj;
 }
 
 // ---- paired form (MC jobs) -------------------------------------------------------------
@@ -1210,39 +1262,44 @@ template <class Sh> hipError_t score_dense_k(const ScoreParams &p, hipStream_t s
   return score_dense_g<Sh::TT, Sh::G, Sh::DS>(p, s);
 }
 
-// the shell-split form exists for tile bounds up to 8 (a shell's tiles fit in LDS)
-template <class Sh> constexpr bool shell_shape() { return Sh::TT <= 8; }
+// the shell-split form: a shell's tiles in LDS at once for tile bounds up to 8 (36 KiB),
+// streamed through LDS in chunks beyond
+template <class Sh> constexpr bool shell_lds_tiles() { return Sh::TT <= 8; }
 
 template <class Sh> hipError_t score_shell_k(const ScoreParams &p, double *vals, hipStream_t s) {
-  if constexpr (!shell_shape<Sh>()) {
-    return hipErrorInvalidValue;
-  } else {
-    const void *fn = reinterpret_cast<const void *>(&score_shell_kernel<Sh::TT, Sh::G>);
-    const hipError_t e = dense_lds(fn, p.wave_lds);
-    if (e != hipSuccess) return e;
-    const uint64_t passes = (p.B + p.dense_cw - 1) / p.dense_cw;
-    const dim3 grid(static_cast<uint32_t>((passes + kShellWaves - 1) / kShellWaves), p.prob.S);
-    hipLaunchKernelGGL((score_shell_kernel<Sh::TT, Sh::G>), grid, dim3(64 * kShellWaves),
-                       p.wave_lds, s, p, vals);
-    const hipError_t r = hipGetLastError();
-    if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(score_sum_kernel, dim3(static_cast<uint32_t>((p.B + 255) / 256)),
-                       dim3(256), 0, s, p, vals);
-    return hipGetLastError();
-  }
+  constexpr bool LT = shell_lds_tiles<Sh>();
+  const void *fn = reinterpret_cast<const void *>(&score_shell_kernel<Sh::TT, Sh::G, Sh::DS, LT>);
+  const hipError_t e = dense_lds(fn, p.wave_lds);
+  if (e != hipSuccess) return e;
+  const dim3 grid(dense_shell_columns(p.B, p.dense_cw), p.prob.S);
+  hipLaunchKernelGGL((score_shell_kernel<Sh::TT, Sh::G, Sh::DS, LT>), grid,
+                     dim3(64 * kShellWaves), p.wave_lds, s, p, vals);
+  const hipError_t r = hipGetLastError();
+  if (r != hipSuccess) return r;
+  hipLaunchKernelGGL(score_sum_kernel, dim3(static_cast<uint32_t>((p.B + kSumCands - 1) / kSumCands)),
+                     dim3(kSumThreads), 0, s, p, vals);
+  return hipGetLastError();
 }
 
 }  // namespace
 
 uint32_t dense_shell_waves() { return kShellWaves; }
 
-bool dense_shell_available(uint32_t tile_bound) {
-  bool ok = false;
+bool dense_shell_available(uint32_t tile_bound) { return dense_groups(tile_bound) > 0; }
+
+uint32_t dense_shell_tile_lds(uint32_t tile_bound) {
+  uint32_t b = 0;
   (void)with_dense_shape(tile_bound, [&](auto sh) {
-    ok = shell_shape<decltype(sh)>();
+    using Sh = decltype(sh);
+    b = (shell_lds_tiles<Sh>() ? Sh::TT * (Sh::TT + 1) / 2 : kShellBufs * kShellChunk) * 1024u;
     return hipSuccess;
   });
-  return ok;
+  return b;
+}
+
+uint32_t dense_shell_columns(uint64_t B, uint32_t cw) {
+  const uint64_t passes = (B + cw - 1) / cw;
+  return static_cast<uint32_t>((passes + kShellWaves - 1) / kShellWaves);
 }
 
 hipError_t launch_score_dense_shells(const ScoreParams &p, uint32_t tile_bound, double *vals,

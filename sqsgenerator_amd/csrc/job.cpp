@@ -1000,13 +1000,14 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
     const uint32_t TT = dense_tile_bound_of(job);
     if (dense_shell_available(TT) && (uint64_t(B) + rp.dense_cw - 1) / rp.dense_cw < want) {
       // a batch too small for a pass of whole candidates per wavefront: one wavefront per
-      // (pass, shell), the blocks sharing each shell's tiles in LDS, the term values in
-      // scratch, then their sums
+      // (pass, shell), the term values in scratch, then their sums
       sp.dense_stride = rp.dense_stride;
-      // a wavefront's candidate rows, later its counts (which need less)
-      sp.shell_rows = (std::max(rp.dense_cw * rp.dense_stride, rp.dense_cw * P.K * P.K * 4) +
-                       15u) & ~15u;
-      sp.wave_lds = TT * (TT + 1) / 2 * 1024 + dense_shell_waves() * sp.shell_rows;
+      // a wavefront's candidate rows, later its counts
+      sp.shell_rows = align(std::max(rp.dense_cw * rp.dense_stride, rp.dense_cw * P.K * P.K * 4));
+      sp.off_row = align(dense_shell_tile_lds(TT) +
+                         P.K * (P.K + 1) / 2 * static_cast<uint32_t>(sizeof(TermParam)));
+      sp.wave_lds = sp.off_row + dense_shell_waves() * sp.shell_rows;
+      if (sp.wave_lds > 160 * 1024) throw std::out_of_range("problem exceeds LDS");
       const size_t bytes = size_t(B) * P.terms.size() * sizeof(double);
       if (job->score_scratch.size() < bytes) job->score_scratch.alloc(bytes);
       SQSG_HIP(launch_score_dense_shells(sp, TT, job->score_scratch.as<double>(),

@@ -48,10 +48,16 @@ hipError_t launch_batch_dense(const BatchParams &bp, uint32_t tile_bound, uint32
 hipError_t prepare_batch_dense(uint32_t tile_bound, size_t lds, bool split = false);
 hipError_t launch_score_dense(const ScoreParams &p, uint32_t tile_bound, hipStream_t stream);
 // explicit scoring split by shell (batches too small to fill the device; tile bounds with
-// dense_shell_available): blocks of dense_shell_waves() wavefronts, one pass of the shape's
-// tile groups each; `vals` is scratch of B x T doubles (the candidates' term values)
+// dense_shell_available): a grid of dense_shell_columns(B, cw) x S blocks of
+// dense_shell_waves() wavefronts, one pass of the shape's tile groups each, then the
+// candidates' sums.  `vals` is scratch of B x T doubles (the candidates' term values;
+// launches that share it must not overlap).  LDS of a block: the shell's tiles or its chunk
+// buffers (dense_shell_tile_lds bytes), its term parameters, then one region of
+// ScoreParams::shell_rows per wavefront at ScoreParams::off_row
 uint32_t dense_shell_waves();
 bool dense_shell_available(uint32_t tile_bound);
+uint32_t dense_shell_tile_lds(uint32_t tile_bound);
+uint32_t dense_shell_columns(uint64_t B, uint32_t cw);
 hipError_t launch_score_dense_shells(const ScoreParams &p, uint32_t tile_bound, double *vals,
                                      hipStream_t stream);
 // paired form (MFMA and helper wavefronts, four pairs per block of 512 threads; MC jobs of

@@ -355,6 +355,26 @@ def test_large_cell_sparse_vs_dense(gpu, name):
     job.close()
 
 
+@pytest.mark.parametrize("name", ["fcc256_quinary", "fcc864_quinary"])
+def test_score_dense_shell_split_consecutive(gpu, name):
+    """explicit scoring split by shell (batches too small to fill the device): consecutive
+    calls of different sizes on one job share its scratch and per-column counters; each call
+    equals the pair-list kernel bit for bit"""
+    print(f"[shell-split-consecutive] {name}", flush=True)
+    cfg = parse_config(WORKLOADS[name]())
+    job = Job(cfg)
+    ref = oracle_inputs(cfg, job)
+    rng = np.random.default_rng(77)
+    cands = random_candidates(ref["packed"], ref["bounds"], 4096, rng)
+    b_w, o_w, s_w = job.score(cands, 0, kernel=1)
+    for B in (4096, 1, 100, 2000, 4096, 37):
+        b_d, o_d, s_d = job.score(cands[:B], 0, kernel=3)
+        assert np.array_equal(b_d, b_w[:B]), B
+        assert np.array_equal(o_d.view(np.uint64), o_w[:B].view(np.uint64)), B
+        assert np.array_equal(s_d, s_w[:B]), B
+    job.close()
+
+
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_dense_matches_single_job_runs(gpu, use_graph):
     """the persistent dense form (jobs of one tile bound share a launch) returns the packs of
