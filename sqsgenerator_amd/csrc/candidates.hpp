@@ -111,6 +111,30 @@ __device__ __forceinline__ bool block_start(const RP &p, uint64_t cand) {
                             : (cand - p.blk_split) % p.blk_size == 0;
 }
 
+// The N-byte LDS rows of the lanes in `pm` (row of lane l at rows + l * row_stride) into
+// bytes [off, off + N) of their record slots (`slot` of each lane, rec_stride bytes per slot;
+// slots at or past `cap` are dropped).  The wavefront copies one row at a time, 4 bytes per
+// lane where the offsets allow (LDS rows are 16-byte aligned), else one byte per lane.
+__device__ __forceinline__ void record_rows(uint8_t *rec_species, uint32_t rec_stride,
+                                            uint32_t cap, unsigned long long pm, uint32_t slot,
+                                            const uint8_t *rows, uint32_t row_stride,
+                                            uint32_t N, uint32_t off, int lane) {
+  const bool words = ((rec_stride | off | N | row_stride) & 3u) == 0;
+  for (unsigned long long m = pm; m; m &= m - 1) {
+    const int l = __ffsll(m) - 1;
+    const uint32_t s = __shfl(slot, l);
+    if (s >= cap) continue;
+    uint8_t *dst = rec_species + static_cast<size_t>(s) * rec_stride + off;
+    const uint8_t *row = rows + static_cast<uint32_t>(l) * row_stride;
+    if (words) {
+      for (uint32_t i = static_cast<uint32_t>(lane); i < N / 4; i += 64)
+        reinterpret_cast<uint32_t *>(dst)[i] = reinterpret_cast<const uint32_t *>(row)[i];
+    } else {
+      for (uint32_t i = static_cast<uint32_t>(lane); i < N; i += 64) dst[i] = row[i];
+    }
+  }
+}
+
 // Bound of the wavefront filter as it stands before a round (it holds only candidates that
 // precede the round's): the order key of its L-th level, or no bound.  The early exit uses it.
 template <class RP>
@@ -129,6 +153,10 @@ __device__ __forceinline__ unsigned long long filter_pass(LevelFilter &lf, const
                                                           unsigned long long mask,
                                                           double tot) {
   if (!p.wave_filter) return mask;
+  // A lane above a full filter's L-th level neither passes nor changes the filter (that
+  // level only decreases as lanes are offered), so only the others are walked: while the
+  // job threshold is still open (a job's first steps) this skips nearly every lane.
+  if (lf.n == p.L) mask &= __ballot(tot <= lf.lvl[p.L - 1]);
   unsigned long long pass = 0;
   while (mask) {
     const int l = __ffsll(mask) - 1;

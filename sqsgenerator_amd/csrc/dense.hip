@@ -68,6 +68,20 @@ __device__ __forceinline__ unsigned long long low_mask(uint32_t n) {
   return n >= 64 ? ~0ull : ((1ull << n) - 1ull);
 }
 
+// Mask of the GEMM2 accumulator registers of a lane that fall on its candidate's diagonal
+// block.  Register r holds row m = 8(r/4) + 4h + r%4 of the lane's column, and the block is
+// rows [cK, cK + K): with `rows` the bit set of those rows, nibble j of the mask is bits
+// 8j + 4h .. 8j + 4h + 3 of it.  Shifts only, no division per register.
+__device__ __forceinline__ uint32_t diag_registers(uint32_t cK, uint32_t K, uint32_t h,
+                                                   bool col_ok) {
+  const uint64_t rows = ((1ull << K) - 1ull) << cK;
+  uint32_t keep = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j)
+    keep |= static_cast<uint32_t>((rows >> (8 * j + 4 * h)) & 0xfull) << (4 * j);
+  return col_ok ? keep : 0u;
+}
+
 // read-only problem data through the constant address space: uniform loads become scalar
 // loads, which do not wait behind the tile stream's vector loads
 template <class T>
@@ -263,12 +277,8 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   const uint32_t cube = pr.S * KK;
   // a spare word past the counts (lanes l and l + 32 share one: its value is never read)
   const uint32_t trash = CW * KK + (static_cast<uint32_t>(lane) & 31u);
-  uint32_t keep = 0;  // accumulator registers of GEMM2 on this lane's diagonal block
-#pragma unroll
-  for (uint32_t r = 0; r < 16; ++r) {
-    const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
-    if (col_ok && m / K == cl) keep |= 1u << r;
-  }
+  // accumulator registers of GEMM2 on this lane's diagonal block
+  const uint32_t keep = diag_registers(cl * K, K, h, col_ok);
   double obj = 0.0;
   // a shell's terms (optimization.h:85-100), added by each candidate's own lane in the
   // reference order
@@ -428,16 +438,17 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
         p.thr_key;
     const bool below = active && okey(tot) <= g;
     const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
+    uint32_t slot = ~0u;
     if (pass) {
-      const uint32_t slot = atomicAdd(p.rec_count, 1u);
+      slot = atomicAdd(p.rec_count, 1u);
       if (slot < p.rec_cap) {
-        uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
-        for (uint32_t i = 0; i < pr.N; ++i) dst[i] = row[i];
         p.rec_cand[slot] = cand;
         p.rec_obj[slot] = tot;
         p.rec_sub[slot] = tot;
       }
     }
+    record_rows(p.rec_species, p.rec_stride, p.rec_cap, __ballot(pass), slot, rows, stride, pr.N,
+                0u, lane);
     wave_sync();  // rows are rewritten by the next round of candidates
   }
 }
@@ -484,19 +495,22 @@ __device__ __forceinline__ void dense_segment_split(const RP &p, uint64_t sbeg, 
   const unsigned long long g = p.thr_key;
   const bool below = active && okey(tot) <= g;
   const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
+  uint32_t slot = ~0u;
   if (pass) {
-    const uint32_t slot = atomicAdd(p.rec_count, 1u);
+    slot = atomicAdd(p.rec_count, 1u);
     if (slot < p.rec_cap) {
-      uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
-      for (uint32_t q = 0; q < nprob; ++q) {
-        const uint8_t *row = wl + p.off_rows[q] + static_cast<uint32_t>(lane) * stride;
-        for (uint32_t i = 0; i < p.prob[q].N; ++i) dst[i] = row[i];
-        dst += p.prob[q].N;
+      for (uint32_t q = 0; q < nprob; ++q)
         p.rec_sub[slot * nprob + q] = subs[q * 64 + static_cast<uint32_t>(lane)];
-      }
       p.rec_cand[slot] = cand;
       p.rec_obj[slot] = tot;
     }
+  }
+  const unsigned long long pm = __ballot(pass);
+  uint32_t off = 0;
+  for (uint32_t q = 0; q < nprob; ++q) {
+    record_rows(p.rec_species, p.rec_stride, p.rec_cap, pm, slot, wl + p.off_rows[q], stride,
+                p.prob[q].N, off, lane);
+    off += p.prob[q].N;
   }
   wave_sync();  // rows are rewritten by the next segment
 }
@@ -1029,17 +1043,17 @@ __device__ __forceinline__ void ws_record(const RP &p, const uint8_t *rows, uint
       p.thr_key;
   const bool below = active && okey(tot) <= g;
   const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
+  uint32_t slot = ~0u;
   if (pass) {
-    const uint32_t slot = atomicAdd(p.rec_count, 1u);
+    slot = atomicAdd(p.rec_count, 1u);
     if (slot < p.rec_cap) {
-      const uint8_t *row = rows + static_cast<uint32_t>(lane) * p.dense_stride;
-      uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
-      for (uint32_t i = 0; i < pr.N; ++i) dst[i] = row[i];
       p.rec_cand[slot] = cand;
       p.rec_obj[slot] = tot;
       p.rec_sub[slot] = tot;
     }
   }
+  record_rows(p.rec_species, p.rec_stride, p.rec_cap, __ballot(pass), slot, rows, p.dense_stride,
+              pr.N, 0u, lane);
 }
 
 // H: starts the pair's share of block batch `bt` in `rows` (sorted rows, shuffle state)

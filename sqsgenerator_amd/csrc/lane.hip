@@ -175,20 +175,28 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
     // comes, cannot be inserted
     const bool below = active && okey(tot) <= g;
     const bool pass = (filter_pass(lf, p, __ballot(below), tot) >> lane) & 1ull;
+    uint32_t slot = ~0u;
     if (pass) {
-      const uint32_t slot = atomicAdd(p.rec_count, 1u);
+      slot = atomicAdd(p.rec_count, 1u);
       if (slot < p.rec_cap) {
-        uint8_t *dst = p.rec_species + static_cast<size_t>(slot) * p.rec_stride;
-        uint32_t off = 0;
-        for (uint32_t q = 0; q < p.nprob; ++q) {
-          const DevProblem &pr = p.prob[q];
-          const uint8_t *col = wl + p.off_rows[q] + lane;
-          for (uint32_t i = 0; i < pr.N; ++i) dst[off + i] = col[i << 6] / pr.F;
-          off += pr.N;
-        }
         p.rec_cand[slot] = cand;
         p.rec_obj[slot] = tot;
         for (uint32_t q = 0; q < p.nprob; ++q) p.rec_sub[slot * p.nprob + q] = sub[q];
+      }
+    }
+    // species of the recorded candidates, one column (candidate) at a time by the wavefront
+    for (unsigned long long m = __ballot(pass); m; m &= m - 1) {
+      const int l = __ffsll(m) - 1;
+      const uint32_t s = __shfl(slot, l);
+      if (s >= p.rec_cap) continue;
+      uint8_t *dst = p.rec_species + static_cast<size_t>(s) * p.rec_stride;
+      uint32_t off = 0;
+      for (uint32_t q = 0; q < p.nprob; ++q) {
+        const DevProblem &pr = p.prob[q];
+        const uint8_t *col = wl + p.off_rows[q] + l;
+        for (uint32_t i = static_cast<uint32_t>(lane); i < pr.N; i += 64)
+          dst[off + i] = col[i << 6] / pr.F;
+        off += pr.N;
       }
     }
   }
@@ -256,7 +264,9 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(BatchParams bp) {
   const RunParams &p = bp.jobs[t];
   if (!p.stage) return;
   const unsigned int count = *p.rec_count;
-  const uint32_t n = min(min(count, p.rec_cap), p.stage_cap);
+  // the host reads the records from the stage only when all of them fit it; more (a job's
+  // first steps) are fetched from the sink
+  const uint32_t n = count <= min(p.rec_cap, p.stage_cap) ? count : 0u;
   uint8_t *st = p.stage;
   if (threadIdx.x == 0) {
     *reinterpret_cast<unsigned int *>(st) = count;
@@ -271,7 +281,11 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(BatchParams bp) {
     obj[k] = p.rec_obj[k];
   }
   for (uint32_t k = threadIdx.x; k < n * p.nprob; k += blockDim.x) sub[k] = p.rec_sub[k];
-  for (uint32_t k = threadIdx.x; k < n * p.rec_stride; k += blockDim.x) sp[k] = p.rec_species[k];
+  // species in 8-byte words (both regions start 8-byte aligned), then the tail bytes
+  const uint32_t bytes = n * p.rec_stride, words = bytes / 8;
+  for (uint32_t k = threadIdx.x; k < words; k += blockDim.x)
+    reinterpret_cast<uint64_t *>(sp)[k] = reinterpret_cast<const uint64_t *>(p.rec_species)[k];
+  for (uint32_t k = 8 * words + threadIdx.x; k < bytes; k += blockDim.x) sp[k] = p.rec_species[k];
 }
 
 // zeroes the item claim counters and every job's record and scored counters (a kernel, so
