@@ -846,8 +846,18 @@ __global__ __launch_bounds__(kSumThreads) void score_sum_kernel(ScoreParams p,
     for (uint32_t k = 0; k < kPer; ++k) v[threadIdx.x + k * kSumThreads] = x[k];
     __syncthreads();
     if (threadIdx.x < kSumCands) {
+      // eight values loaded together, then added one after another in order
       const uint32_t nt = T - t0 < kSumTerms ? T - t0 : kSumTerms;
-      for (uint32_t t = 0; t < nt; ++t) obj += v[t * kSumCands + threadIdx.x];
+      const double *w = v + threadIdx.x;
+      uint32_t t = 0;
+      for (; t + 8 <= nt; t += 8) {
+        double x[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) x[k] = w[(t + k) * kSumCands];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) obj += x[k];
+      }
+      for (; t < nt; ++t) obj += w[t * kSumCands];
     }
     __syncthreads();  // the next chunk overwrites the stage
   }
