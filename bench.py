@@ -220,6 +220,47 @@ def large_cell_scoring(launches: int, batch: int = 4096) -> dict:
     return out
 
 
+def autotuned_runs(iterations: int) -> dict:
+    """single-job sqsg_run of the benchmark cells at the default launch parameters and after
+    sqsg_autotune (variant, dense launch shape, candidate passes per round, launch size):
+    candidates per second of the run's wall clock (best of three runs), tuning time apart"""
+    import sqsgenerator_amd._lib as L
+    import workloads as W
+    from sqsgenerator_amd.core import Job, parse_config
+
+    cells = [("a_fcc108_ternary", W.ternary_108, iterations),
+             ("b_bcc128_quaternary", W.quaternary_bcc_128, iterations),
+             ("c_hcp128_binary", W.hcp_128, iterations),
+             ("fcc256_quinary", W.quinary_256, iterations),
+             ("fcc864_quinary", W.quinary_864, iterations // 8)]
+
+    def run(job, repeats=3):  # the best of three runs
+        best = 0.0
+        for _ in range(repeats):
+            L.check(L.lib().sqsg_device_synchronize())
+            t0 = time.perf_counter()
+            job.run()
+            best = max(best, job.timing()[2] / (time.perf_counter() - t0))
+        return best
+
+    out = {"iterations": iterations}
+    for name, make, its in cells:
+        job = Job(parse_config(make(iterations=its)))
+        run(job, 1)  # inputs resident, kernels loaded
+        default = run(job)
+        shape0 = job.kernel_shape()
+        t0 = time.perf_counter()
+        kernel, _ = job.autotune()
+        tune_s = time.perf_counter() - t0
+        tuned = run(job)
+        out[name] = {"candidates": its, "default_candidates_per_s": default,
+                     "default_shape": shape0, "tuned_candidates_per_s": tuned,
+                     "tuned_shape": job.kernel_shape(), "tuned_launch": job.dense_launch(),
+                     "tuning_s": tune_s, "speedup": tuned / default}
+        job.close()
+    return out
+
+
 def batched_uneven(iterations: int) -> dict:
     """configurations (a)-(d): FCC 108 ternary, BCC 128 quaternary, HCP 128 binary and the
     rocksalt 216 split job, `iterations` MC candidates each, in one batched run (inputs
@@ -367,6 +408,9 @@ def main() -> int:
     ap.add_argument("--uneven-iterations", type=int, default=10_000_000,
                     help="MC candidates per job of the batched uneven leg (configs (a)-(d)); "
                          "0: skip it")
+    ap.add_argument("--autotune-iterations", type=int, default=16_000_000,
+                    help="MC candidates per single-job run of the autotuner leg (1/8 of it at "
+                         "864 sites); 0: skip it")
     ap.add_argument("--no-prune", action="store_true",
                     help="skip the secondary run with early exit (sqsg_job_set_pruning)")
     ap.add_argument("--serve-sweep", default="10,30,100,300,1000",
@@ -512,6 +556,8 @@ def main() -> int:
         out["large_cell_scoring"] = large_cell_scoring(args.large_cell_launches)
     if args.uneven_iterations:
         out["batched_uneven"] = batched_uneven(args.uneven_iterations)
+    if args.autotune_iterations:
+        out["autotune"] = autotuned_runs(args.autotune_iterations)
     if not args.no_serve:
         out["serving"] = serving(args.serve_rate, args.serve_seconds, 20261203,
                                  args.serve_max_budget)

@@ -238,9 +238,25 @@ typedef struct sqsg_batch_report {
  * `use_graph`); each job's results are then read with sqsg_result_*. */
 SQSG_API int sqsg_batch_run(sqsg_job **jobs, uint32_t num_jobs, int use_graph,
                             sqsg_batch_report *report);
-/* Autotuner: times each kernel variant on a short launch of `job` and selects the fastest
- * (written to *chosen, also stored in the job). */
+/* Autotuner (no reference counterpart): times the job's applicable kernel variants on a probe
+ * of its own candidates and, for the dense kernel of a one-problem job, its launch
+ * parameters: the alternative launch shapes of its tile bound (tile groups, ring depth), one
+ * pass of candidates fewer per wavefront round, then the candidates per sqsg_run launch
+ * (unless the caller set a launch size).  Keeps the fastest in the job (variant written to
+ * *chosen, its probe rate to *cand_per_s; sqsg_job_kernel_shape and sqsg_job_dense_launch
+ * report the rest).  Results are identical for every choice; the probes overwrite the job's
+ * results. */
 SQSG_API int sqsg_autotune(sqsg_job *job, int *chosen, double *cand_per_s);
+/* Dense search launch parameters of a job (no reference counterpart; what the autotuner
+ * chooses, settable to reproduce a tuned run): `shape` 0 is the tile bound's default launch
+ * shape and 1 .. num_shapes - 1 its alternatives (one-problem jobs), `passes` the candidates
+ * per wavefront round in passes of the shape's width (0: as many as its LDS budget holds),
+ * `launch` the candidates per sqsg_run launch (0: the default; a launch size set with
+ * sqsg_job_set_launch_size takes precedence).  Results are identical for every setting. */
+SQSG_API int sqsg_job_set_dense_launch(sqsg_job *job, uint32_t shape, uint32_t passes,
+                                       uint64_t launch);
+SQSG_API int sqsg_job_dense_launch(sqsg_job *job, uint32_t *num_shapes, uint32_t *shape,
+                                   uint32_t *passes, uint64_t *launch);
 /* Launch shape of the kernel a run of `job` uses (no reference counterpart: introspection
  * for profiling and reports), e.g. "dense<8,2,2,3>" (tile bound, tile groups, wavefronts
  * per SIMD, ring depth), "paired<8,2,9>", "lane<5>" or "wave"; NUL-terminated, truncated

@@ -131,6 +131,8 @@ SIGNATURES = {
     "sqsg_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), f64p]),
     "sqsg_job_kernel_shape": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "sqsg_job_set_pruning": (C.c_int, [C.c_void_p, C.c_int]),
+    "sqsg_job_set_dense_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64]),
+    "sqsg_job_dense_launch": (C.c_int, [C.c_void_p, u32p, u32p, u32p, u64p]),
     "sqsg_server_create": (C.c_int, [C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]),
     "sqsg_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, u64p]),
     "sqsg_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -758,10 +758,23 @@ class Job:
         return buf.value.decode()
 
     def autotune(self) -> tuple[int, float]:
-        """times every kernel variant on a short run and keeps the fastest (clears results)"""
+        """times the kernel variants and the dense launch parameters on probes of the job and
+        keeps the fastest (clears results); returns the variant and its probe rate"""
         k, r = C.c_int(), C.c_double()
         L.check(L.lib().sqsg_autotune(self.handle, C.byref(k), C.byref(r)))
         return k.value, r.value
+
+    def set_dense_launch(self, shape: int = 0, passes: int = 0, launch: int = 0) -> None:
+        """dense search launch shape (0: default, 1..: alternatives of the tile bound),
+        candidate passes per wavefront round (0: as many as fit) and candidates per launch
+        (0: default); results are identical for every setting"""
+        L.check(L.lib().sqsg_job_set_dense_launch(self.handle, shape, passes, launch))
+
+    def dense_launch(self) -> dict:
+        n, s, p, l = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        L.check(L.lib().sqsg_job_dense_launch(self.handle, C.byref(n), C.byref(s), C.byref(p),
+                                              C.byref(l)))
+        return dict(num_shapes=n.value, shape=s.value, passes=p.value, launch=l.value)
 
 
 def batch_run(jobs: list["Job"], use_graph: bool = True) -> dict:

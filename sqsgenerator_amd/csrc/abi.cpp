@@ -8,6 +8,7 @@
 #include "../../include/sqsgen_amd.h"
 #include "geometry.hpp"
 #include "job.hpp"
+#include "kernels.hpp"
 #include "permutation.hpp"
 
 namespace {
@@ -397,6 +398,34 @@ int sqsg_job_kernel_shape(sqsg_job *job, char *buf, size_t len) {
     const size_t n = std::min(len - 1, s.size());
     std::memcpy(buf, s.data(), n);
     buf[n] = '\0';
+  });
+}
+
+int sqsg_job_set_dense_launch(sqsg_job *job, uint32_t shape, uint32_t passes, uint64_t launch) {
+  return guard([&] {
+    need(job, "job");
+    sqsg::analyze_job(job);
+    if (shape && (job->problems.size() != 1 ||
+                  shape > sqsg::dense_alt_shapes(sqsg::dense_tile_bound_of(job))))
+      throw std::out_of_range("no such alternative dense launch shape for this job");
+    job->dense_alt = shape;
+    job->dense_passes = passes;
+    job->tuned_launch = launch;
+  });
+}
+
+int sqsg_job_dense_launch(sqsg_job *job, uint32_t *num_shapes, uint32_t *shape,
+                          uint32_t *passes, uint64_t *launch) {
+  return guard([&] {
+    need(job, "job");
+    sqsg::analyze_job(job);
+    if (num_shapes)
+      *num_shapes = job->dense_ok && job->problems.size() == 1
+                        ? 1 + sqsg::dense_alt_shapes(sqsg::dense_tile_bound_of(job))
+                        : (job->dense_ok ? 1u : 0u);
+    if (shape) *shape = job->dense_alt;
+    if (passes) *passes = job->dense_passes;
+    if (launch) *launch = job->tuned_launch;
   });
 }
 

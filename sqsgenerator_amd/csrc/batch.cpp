@@ -12,7 +12,6 @@
 // merges its records, tightens each job's threshold and admits or retires jobs (which is
 // what the serving loop builds on); the device never waits for the host between steps.
 #include <algorithm>
-#include <cstring>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -106,6 +105,7 @@ struct Group {
   int variant = 0;
   uint32_t key = 0;    // the group's identity within its variant
   uint32_t shape = 0;  // launch template: lane: largest species count; dense: tile bound
+  uint32_t alt = 0;    // dense: the alternative shape of the tile bound (0: default)
   DevBuf d_jobs, d_items, d_counts, d_claims;
   hipGraphExec_t exec = nullptr;
   uint32_t blocks = 0;     // of the current step's launch
@@ -236,12 +236,15 @@ class BatchEngine {
     // groups share a launch shape: the tile bound (dense), also K (paired: one LDS
     // layout), the largest species count (lane); split jobs have dense kernels of their own
     const bool split = job->problems.size() > 1;
+    // (and the tuned alternative shape)
+    const uint32_t alt = variant == SQSG_KERNEL_DENSE_MFMA ? search_alt(job) : 0u;
     const uint32_t key = !dense ? job->kmax
                          : variant == SQSG_KERNEL_DENSE_PAIRED
                              ? dense_tile_bound_of(job) | (job->problems[0].K << 8)
-                             : dense_tile_bound_of(job) | (split ? 1u << 16 : 0u);
+                             : dense_tile_bound_of(job) | (split ? 1u << 16 : 0u) | (alt << 20);
     Group &g = group(variant, key, dense ? dense_tile_bound_of(job) : job->kmax);
     g.split = dense && split;
+    g.alt = alt;
     if (g.members.size() >= kGroupJobs) throw std::out_of_range("too many resident jobs");
     g.members.push_back(s.get());
     s->group = &g;
@@ -494,7 +497,8 @@ class BatchEngine {
     uint32_t per_cu = static_cast<uint32_t>((160 * 1024) / std::max<size_t>(block_lds, 1));
     per_cu = std::max(1u, std::min(per_cu, 16u));
     if (g.variant == SQSG_KERNEL_DENSE_MFMA)
-      per_cu = dense_waves_per_cu(g.shape, block_lds, true, g.split);
+      per_cu = g.alt ? dense_alt_waves_per_cu(g.shape, g.alt, block_lds, true)
+                     : dense_waves_per_cu(g.shape, block_lds, true, g.split);
     else if (g.variant == SQSG_KERNEL_DENSE_PAIRED)
       per_cu = dense_paired_blocks_per_cu(g.shape, block_lds, true);
     const uint32_t blocks = static_cast<uint32_t>(device_cu_count()) * per_cu;
@@ -508,7 +512,9 @@ class BatchEngine {
     bp.claims = g.d_claims.as<unsigned int>();
     bp.max_items = kGroupItems;
     g.blocks = blocks;
-    SQSG_HIP(g.variant == SQSG_KERNEL_DENSE_MFMA     ? prepare_batch_dense(g.shape, block_lds, g.split)
+    SQSG_HIP(g.variant == SQSG_KERNEL_DENSE_MFMA
+                 ? (g.alt ? prepare_batch_dense_alt(g.shape, g.alt, block_lds)
+                          : prepare_batch_dense(g.shape, block_lds, g.split))
              : g.variant == SQSG_KERNEL_DENSE_PAIRED ? prepare_batch_dense_paired(g.shape, block_lds)
                                                      : prepare_batch_lane(g.shape, block_lds));
     if (use_graph_ && (!g.exec || g.graph_wave_lds != wave_lds || g.graph_blocks != blocks)) {
@@ -539,7 +545,8 @@ class BatchEngine {
   hipError_t launch_kernel(const Group &g, hipStream_t st) {
     switch (g.variant) {
       case SQSG_KERNEL_DENSE_MFMA:
-        return launch_batch_dense(g.bp, g.shape, g.blocks, st, g.split);
+        return g.alt ? launch_batch_dense_alt(g.bp, g.shape, g.alt, g.blocks, st)
+                     : launch_batch_dense(g.bp, g.shape, g.blocks, st, g.split);
       case SQSG_KERNEL_DENSE_PAIRED:
         return launch_batch_dense_paired(g.bp, g.shape, g.blocks, st);
       default: return launch_batch_lane(g.bp, g.shape, g.blocks, 1, st);
@@ -604,40 +611,92 @@ void batch_run(sqsg_job **jobs, uint32_t n, int use_graph, sqsg_batch_report *re
   }
 }
 
-// Times every applicable kernel variant on a short launch of the job and keeps the fastest.
-// The probe runs overwrite the job's results.
+// Autotuner: times the job's applicable kernel variants on a probe of its own candidates and,
+// for the dense kernel of a one-problem job, its launch parameters: the alternative launch
+// shapes of the tile bound (tile groups, ring depth), one pass fewer per wavefront round
+// (candidates per round), and then the single-job launch size (candidates per launch: the
+// default and 4x and 16x of it, as long as a launch stays near 50 ms).  Each configuration is
+// run once to warm up and once timed (wall clock of the run, host merges included); the
+// fastest is kept in the job.  Results do not depend on the choice: candidate t is the same
+// configuration in every kernel and shape, and records merge in candidate order.  The probe
+// runs overwrite the job's results.
 void autotune(sqsg_job *job, int *chosen, double *rate) {
-  ensure_device(job);
-  std::vector<int> variants{SQSG_KERNEL_SPARSE_WAVE};
-  if (job->lane_ok) variants.push_back(SQSG_KERNEL_SPARSE_LANE);
-  if (job->dense_ok) variants.push_back(SQSG_KERNEL_DENSE_MFMA);
-  if (paired_ok(job)) variants.push_back(SQSG_KERNEL_DENSE_PAIRED);
+  ensure_device(job);  // analyses the job's kernel applicability too
+  struct Config {
+    int variant;
+    uint32_t alt, passes;
+  };
+  std::vector<Config> configs{{SQSG_KERNEL_SPARSE_WAVE, 0, 0}};
+  if (job->lane_ok) configs.push_back({SQSG_KERNEL_SPARSE_LANE, 0, 0});
   const uint64_t saved_it = job->iterations, saved_launch = job->launch_size;
   const int saved_kernel = job->kernel;
+  const uint32_t saved_alt = job->dense_alt, saved_passes = job->dense_passes;
+  const uint64_t saved_tuned = job->tuned_launch;
+  auto restore = [&] {
+    job->iterations = saved_it, job->launch_size = saved_launch, job->kernel = saved_kernel;
+    job->dense_alt = saved_alt, job->dense_passes = saved_passes;
+    job->tuned_launch = saved_tuned;
+  };
+  job->dense_alt = 0, job->dense_passes = 0, job->tuned_launch = 0;
+  if (job->dense_ok) {
+    configs.push_back({SQSG_KERNEL_DENSE_MFMA, 0, 0});
+    if (job->problems.size() == 1) {
+      const uint32_t TT = dense_tile_bound_of(job);
+      for (uint32_t a = 1; a <= dense_alt_shapes(TT); ++a)
+        configs.push_back({SQSG_KERNEL_DENSE_MFMA, a, 0});
+      uint32_t blocks = 0;
+      const RunParams p = make_run_params(job, SQSG_KERNEL_DENSE_MFMA, 1, &blocks);
+      const uint32_t npass = p.gen_group / p.dense_cw;
+      if (npass > 1) configs.push_back({SQSG_KERNEL_DENSE_MFMA, 0, npass - 1});
+    }
+  }
+  if (paired_ok(job)) configs.push_back({SQSG_KERNEL_DENSE_PAIRED, 0, 0});
   uint64_t work = 0;
   for (const auto &P : job->problems) work += P.pi.size() + 64;
-  const uint64_t probe = std::clamp<uint64_t>((uint64_t(1) << 31) / work, 4096, saved_it);
-  int best = SQSG_KERNEL_SPARSE_WAVE;
+  // about 2^35 pair visits: 2M candidates at 256 sites, a few ms on the dense kernel
+  const uint64_t probe = std::clamp<uint64_t>((uint64_t(1) << 35) / work, 4096, saved_it);
+  auto timed = [&](uint64_t iterations, uint64_t launch) {
+    job->iterations = iterations;
+    job->launch_size = launch;
+    const auto t0 = std::chrono::steady_clock::now();
+    run_job(job, nullptr, nullptr);
+    const double s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return static_cast<double>(job->scored) / std::max(s, 1e-9);
+  };
+  Config best = configs[0];
   double best_rate = 0;
   try {
-    job->iterations = probe;
-    job->launch_size = probe;
-    for (int v : variants) {
-      job->kernel = v;
-      run_job(job, nullptr, nullptr);  // warm-up (module load, first-touch)
-      run_job(job, nullptr, nullptr);
-      const double r = job->scored / (job->kernel_ms * 1e-3);
-      if (r > best_rate) best_rate = r, best = v;
+    for (const Config &c : configs) {
+      job->kernel = c.variant, job->dense_alt = c.alt, job->dense_passes = c.passes;
+      (void)timed(probe, probe);  // warm-up (module load, first touch)
+      const double r = timed(probe, probe);
+      if (r > best_rate) best_rate = r, best = c;
+    }
+    job->kernel = best.variant, job->dense_alt = best.alt, job->dense_passes = best.passes;
+    if (!saved_launch && saved_it > default_launch_size(job)) {
+      // launch size of single-job runs: each launch ends with its slowest wavefront and a
+      // host merge, so longer launches waste less, up to ~50 ms per launch
+      const uint64_t base = default_launch_size(job);
+      double best_l = 0;
+      uint64_t pick = 0;
+      for (uint64_t f : {1, 4, 16}) {
+        const uint64_t L = base * f;
+        if (f > 1 && (L > saved_it / 2 || static_cast<double>(L) / best_rate > 0.05)) break;
+        const double r = timed(std::min<uint64_t>(3 * L, saved_it), L);
+        if (r > best_l) best_l = r, pick = f > 1 ? L : 0;
+      }
+      job->tuned_launch = pick;
+      best_rate = std::max(best_rate, best_l);
     }
   } catch (...) {
-    job->iterations = saved_it, job->launch_size = saved_launch, job->kernel = saved_kernel;
+    restore();
     throw;
   }
   job->iterations = saved_it;
   job->launch_size = saved_launch;
-  job->kernel = best;
   reset_results(job);
-  if (chosen) *chosen = best;
+  if (chosen) *chosen = best.variant;
   if (rate) *rate = best_rate;
 }
 

@@ -545,8 +545,14 @@ sqsg_job *create_job(const sqsg_config &c) {
 std::string kernel_shape(sqsg_job *job) {
   analyze_job(job);
   switch (resolve_kernel(job, job->kernel)) {
-    case SQSG_KERNEL_DENSE_MFMA:
-      return "dense<" + dense_shape_string(dense_tile_bound_of(job), false) + ">";
+    case SQSG_KERNEL_DENSE_MFMA: {
+      const uint32_t TT = dense_tile_bound_of(job), alt = search_alt(job);
+      std::string s = "dense<" +
+                      (alt ? dense_alt_shape_string(TT, alt) : dense_shape_string(TT, false)) +
+                      ">";
+      if (job->dense_passes) s += " passes " + std::to_string(job->dense_passes);
+      return s;
+    }
     case SQSG_KERNEL_DENSE_PAIRED:
       return "paired<" + dense_shape_string(dense_tile_bound_of(job), true) + ">";
     case SQSG_KERNEL_SPARSE_LANE:
@@ -607,7 +613,12 @@ uint32_t first_sink_records(const sqsg_job *job, uint64_t waves, uint64_t launch
   return static_cast<uint32_t>(std::min<uint64_t>({launch, want, uint64_t(1) << 22}));
 }
 
-RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *blocks) {
+uint32_t search_alt(const sqsg_job *job) {
+  return job->problems.size() == 1 ? job->dense_alt : 0u;
+}
+
+RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *blocks,
+                          bool search) {
   RunParams p{};
   p.nprob = static_cast<uint32_t>(job->problems.size());
   uint32_t stride_sum = 0, max_bins = 0, max_terms = 0;
@@ -664,7 +675,8 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
       T = std::max(T, (P.N + 31) / 32);
       Kmax = std::max(Kmax, P.K);
     }
-    const uint32_t TT = dense_tile_bound(T), G = dense_groups(TT);
+    const uint32_t TT = dense_tile_bound(T), alt = search ? search_alt(job) : 0u;
+    const uint32_t G = alt ? dense_alt_groups(TT, alt) : dense_groups(TT);
     const uint32_t g = std::min(32 / Kmax, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
@@ -680,6 +692,7 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     // at most 20 KiB per wavefront: eight resident wavefronts per CU (two per SIMD)
     while (npass > 1 && rows_bytes(npass) + cnt_bytes + obj_bytes + lf_bytes > 20 * 1024)
       --npass;
+    if (search && job->dense_passes) npass = std::min(npass, job->dense_passes);
     p.gen_group = npass * p.dense_cw;
     const uint32_t rb = align(p.gen_group * p.dense_stride);
     for (uint32_t q = 0; q < nprob; ++q) p.off_rows[q] = q * rb;
@@ -691,7 +704,8 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     p.prune = job->prune && nprob == 1 ? 1u : 0u;  // the early exit bounds one problem's terms
     // one wavefront per block, as many as registers and LDS keep resident
     *blocks = static_cast<uint32_t>(device_cu_count()) *
-              dense_waves_per_cu(TT, p.wave_lds, false, nprob > 1);
+              (alt ? dense_alt_waves_per_cu(TT, alt, p.wave_lds, false)
+                   : dense_waves_per_cu(TT, p.wave_lds, false, nprob > 1));
     return p;
   }
   if (variant == SQSG_KERNEL_SPARSE_LANE) {
@@ -729,8 +743,11 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
 hipError_t launch_variant(const RunParams &p, const sqsg_job *job, int variant, uint32_t blocks,
                           uint32_t W, hipStream_t s) {
   if (variant == SQSG_KERNEL_SPARSE_LANE) return launch_run_lane(p, job->kmax, blocks, W, s);
-  if (variant == SQSG_KERNEL_DENSE_MFMA)
+  if (variant == SQSG_KERNEL_DENSE_MFMA) {
+    if (const uint32_t alt = search_alt(job))
+      return launch_run_dense_alt(p, dense_tile_bound_of(job), alt, blocks, s);
     return launch_run_dense(p, dense_tile_bound_of(job), blocks, s, job->problems.size() > 1);
+  }
   if (variant == SQSG_KERNEL_DENSE_PAIRED)
     return launch_run_dense_paired(p, dense_tile_bound_of(job), blocks, s);
   return launch_run_wave(p, blocks, W, s);
@@ -872,7 +889,9 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user, const FixedScratch 
   const uint32_t W = variant == SQSG_KERNEL_SPARSE_WAVE ? 4 : 1;
   uint32_t blocks = 0;
   RunParams p = make_run_params(job, variant, W, &blocks);
-  const uint64_t launch = job->launch_size ? job->launch_size : default_launch_size(job);
+  const uint64_t launch = job->launch_size    ? job->launch_size
+                          : job->tuned_launch ? job->tuned_launch
+                                              : default_launch_size(job);
   const uint32_t stride = job->stride;
   p.rec_stride = stride;
   RecordSink sink;
@@ -992,7 +1011,7 @@ void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
     ensure_dense(job);
     uint32_t blocks = 0;
-    const RunParams rp = make_run_params(job, SQSG_KERNEL_DENSE_MFMA, 1, &blocks);
+    const RunParams rp = make_run_params(job, SQSG_KERNEL_DENSE_MFMA, 1, &blocks, false);
     sp.prob = P.dev();
     sp.dense_cw = rp.dense_cw;
     sp.dense_groups = dense_groups(dense_tile_bound_of(job));

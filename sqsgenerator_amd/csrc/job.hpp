@@ -170,6 +170,12 @@ struct sqsg_job {
   bool prune = false;     // early exit of hopeless passes (dense kernel, pw >= 0)
   sqsg::DevBuf score_scratch;  // term values of explicit scoring split by shell
   uint64_t launch_size = 0;
+  // launch parameters chosen by the autotuner for the dense search kernels of a job of one
+  // problem: an alternative launch shape of the tile bound (0: the default), candidates per
+  // wavefront round in passes of the shape's width (0: as many as its LDS budget holds), and
+  // single-job candidates per launch (0: default_launch_size; an explicit launch size wins)
+  uint32_t dense_alt = 0, dense_passes = 0;
+  uint64_t tuned_launch = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool on_device = false;
@@ -217,8 +223,11 @@ int guarded(const std::function<void()> &fn);
 void score_batch(sqsg_job *job, uint32_t problem, const uint8_t *d_species, uint32_t B,
                  int32_t *d_bonds, double *d_obj, double *d_sro, int kernel, hipStream_t s);
 // helpers shared with the batched and serving drivers
+// (search: with the job's tuned launch parameters; explicit scoring uses the defaults)
 RunParams make_run_params(sqsg_job *job, int variant, uint32_t waves_per_block,
-                          uint32_t *blocks);
+                          uint32_t *blocks, bool search = true);
+// the tuned alternative dense shape a search of the job runs (0: the default shape)
+uint32_t search_alt(const sqsg_job *job);
 // Device record sink of one job: candidates at or below the threshold with their
 // objectives and packed species.  Its arrays are its own allocations, or memory the caller
 // lends it (place).
@@ -285,6 +294,8 @@ hipStream_t job_stream(sqsg_job *job);
 void ensure_dense(sqsg_job *job);
 // dense variant launch shape of a job: tile bound (kernel template) and 1-wave blocks
 uint32_t dense_tile_bound_of(const sqsg_job *job);
+// which kernel variants apply to the job (host data; kmax, lane_ok, dense_ok)
+void analyze_job(sqsg_job *job);
 // the paired dense form applies (Monte Carlo, a tile bound with a paired shape)
 bool paired_ok(const sqsg_job *job);
 // launch shape of the kernel a run of the job uses ("dense<TT,G,W,D>", "lane<K>", ...)

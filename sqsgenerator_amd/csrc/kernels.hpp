@@ -70,6 +70,19 @@ uint32_t dense_paired_blocks_per_cu(uint32_t tile_bound, size_t block_lds, bool 
 hipError_t launch_run_dense_paired(const RunParams &p, uint32_t tile_bound, uint32_t blocks,
                                    hipStream_t stream);
 hipError_t prepare_batch_dense_paired(uint32_t tile_bound, size_t block_lds);
+// alternative search shapes of a tile bound (dense_alt.hip), numbered from 1 (0 is the
+// default shape above): the same tile bound with other tile groups or ring depths, for jobs
+// of one problem; the autotuner times them against the default
+uint32_t dense_alt_shapes(uint32_t tile_bound);
+uint32_t dense_alt_groups(uint32_t tile_bound, uint32_t alt);
+std::string dense_alt_shape_string(uint32_t tile_bound, uint32_t alt);
+uint32_t dense_alt_waves_per_cu(uint32_t tile_bound, uint32_t alt, size_t wave_lds,
+                                bool persistent);
+hipError_t launch_run_dense_alt(const RunParams &p, uint32_t tile_bound, uint32_t alt,
+                                uint32_t blocks, hipStream_t stream);
+hipError_t prepare_batch_dense_alt(uint32_t tile_bound, uint32_t alt, size_t lds);
+hipError_t launch_batch_dense_alt(const BatchParams &bp, uint32_t tile_bound, uint32_t alt,
+                                  uint32_t blocks, hipStream_t stream);
 hipError_t launch_batch_dense_paired(const BatchParams &bp, uint32_t tile_bound, uint32_t blocks,
                                      hipStream_t stream);
 

@@ -333,6 +333,65 @@ def test_autotune_picks_a_variant(gpu):
     assert job.timing()[2] == 131072
 
 
+@pytest.mark.parametrize("name", ["fcc32_3shells", "fcc64_ternary", "fcc108_ternary",
+                                  "fcc256_quinary", "fcc864_quinary"])
+def test_dense_launch_shapes_same_packs(gpu, name):
+    """every alternative dense launch shape of the cell's tile bound, fewer candidate passes
+    per wavefront round and other launch sizes return the default shape's pack, in single-job
+    runs and side by side in one batched run; the pack is re-scored by the oracle"""
+    its = 20_000 if name == "fcc864_quinary" else 100_000
+    d = _small(name, its)
+    ref_job = Job(parse_config(d))
+    ref_job.set_kernel(3)
+    ref_job.run()
+    ref = _pack_signature(ref_job.results())
+    assert Rescorer(ref_job.config).check_pack(ref_job.results()) > 0
+    n = ref_job.dense_launch()["num_shapes"]
+    assert n >= 2, "every tile bound of these cells has an alternative shape"
+    ref_job.close()
+    settings = [(a, 0, 0) for a in range(1, n)] + [(0, 1, 0), (0, 2, 7_001), (n - 1, 1, 30_000)]
+    jobs = []
+    for shape, passes, launch in settings:
+        print(f"[shapes] {name} shape={shape} passes={passes} launch={launch}", flush=True)
+        job = Job(parse_config(d))
+        job.set_kernel(3)
+        job.set_dense_launch(shape, passes, launch)
+        assert job.dense_launch()["shape"] == shape
+        job.run()
+        assert job.timing()[2] == its
+        assert _pack_signature(job.results()) == ref, (shape, passes, launch)
+        job.close()
+        b = Job(parse_config(d))
+        b.set_kernel(3)
+        b.set_dense_launch(shape, passes, 0)
+        b.set_launch_size(its // 3 + 1)
+        jobs.append(b)
+    S.batch_run(jobs)
+    for (shape, passes, _), b in zip(settings, jobs):
+        assert _pack_signature(b.results()) == ref, ("batched", shape, passes)
+        b.close()
+
+
+def test_autotune_chooses_launch_parameters(gpu):
+    """the autotuner keeps a variant, a dense shape, a pass count and a launch size; a run
+    with them returns the untuned run's pack"""
+    d = _small("fcc256_quinary", 3_000_000)
+    ref_job = Job(parse_config(d))
+    ref_job.run()
+    ref = _pack_signature(ref_job.results())
+    ref_job.close()
+    job = Job(parse_config(d))
+    kernel, rate = job.autotune()
+    tuned = job.dense_launch()
+    print(f"[autotune] kernel={kernel} rate={rate:.3g} {job.kernel_shape()} {tuned}", flush=True)
+    assert kernel in (1, 2, 3, 4) and rate > 0
+    assert 0 <= tuned["shape"] < tuned["num_shapes"]
+    job.run()
+    assert job.timing()[2] == 3_000_000
+    assert _pack_signature(job.results()) == ref
+    job.close()
+
+
 @pytest.mark.parametrize("name", ["fcc256_quinary", "fcc864_quinary"])
 def test_large_cell_sparse_vs_dense(gpu, name):
     """4096 uniformly random candidates scored by the LDS pair-list kernel and by the int8
