@@ -193,7 +193,7 @@ class BatchEngine {
     s->job = job;
     uint32_t blocks = 0;
     s->p = make_run_params(job, variant, 1, &blocks);
-    s->slice = job->launch_size ? job->launch_size : default_launch_size(job);
+    s->slice = job->launch_size ? job->launch_size : default_step_size(job);
     uint64_t work = 0;
     for (const auto &P : job->problems) work += P.pi.size() + 64;
     const uint64_t gen = s->p.gen_group;
@@ -611,13 +611,14 @@ void batch_run(sqsg_job **jobs, uint32_t n, int use_graph, sqsg_batch_report *re
   }
 }
 
-// Autotuner: times the job's applicable kernel variants on a probe of its own candidates and,
+// Autotuner: times the job's applicable kernel variants on probes of its own candidates and,
 // for the dense kernel of a one-problem job, its launch parameters: the alternative launch
 // shapes of the tile bound (tile groups, ring depth), one pass fewer per wavefront round
 // (candidates per round), and then the single-job launch size (candidates per launch: the
 // default and 4x and 16x of it, as long as a launch stays near 50 ms).  Each configuration is
-// run once to warm up and once timed (wall clock of the run, host merges included); the
-// fastest is kept in the job.  Results do not depend on the choice: candidate t is the same
+// run once to warm up and once timed over two of its launches (wall clock of the run, host
+// merges included); the fastest is kept in the job, an alternative of the same variant only
+// when it is more than 1% faster.  Results do not depend on the choice: candidate t is the same
 // configuration in every kernel and shape, and records merge in candidate order.  The probe
 // runs overwrite the job's results.
 void autotune(sqsg_job *job, int *chosen, double *rate) {
@@ -651,10 +652,11 @@ void autotune(sqsg_job *job, int *chosen, double *rate) {
     }
   }
   if (paired_ok(job)) configs.push_back({SQSG_KERNEL_DENSE_PAIRED, 0, 0});
-  uint64_t work = 0;
-  for (const auto &P : job->problems) work += P.pi.size() + 64;
-  // about 2^35 pair visits: 2M candidates at 256 sites, a few ms on the dense kernel
-  const uint64_t probe = std::clamp<uint64_t>((uint64_t(1) << 35) / work, 4096, saved_it);
+  // each configuration runs with the launch size a run of it would use (the caller's, or
+  // the variant's default) over about two of those launches
+  auto launch_of = [&]() {
+    return saved_launch ? saved_launch : default_launch_size(job);
+  };
   auto timed = [&](uint64_t iterations, uint64_t launch) {
     job->iterations = iterations;
     job->launch_size = launch;
@@ -669,9 +671,13 @@ void autotune(sqsg_job *job, int *chosen, double *rate) {
   try {
     for (const Config &c : configs) {
       job->kernel = c.variant, job->dense_alt = c.alt, job->dense_passes = c.passes;
-      (void)timed(probe, probe);  // warm-up (module load, first touch)
-      const double r = timed(probe, probe);
-      if (r > best_rate) best_rate = r, best = c;
+      const uint64_t launch = std::min(launch_of(), saved_it);
+      const uint64_t probe = std::min<uint64_t>(2 * launch, saved_it);
+      (void)timed(probe, launch);  // warm-up (module load, first touch)
+      const double r = timed(probe, launch);
+      // an alternative of the same variant must win by more than the timing noise
+      const bool same_variant = c.variant == best.variant && (c.alt || c.passes);
+      if (r > best_rate * (same_variant ? 1.01 : 1.0)) best_rate = r, best = c;
     }
     job->kernel = best.variant, job->dense_alt = best.alt, job->dense_passes = best.passes;
     if (!saved_launch && saved_it > default_launch_size(job)) {
@@ -684,7 +690,7 @@ void autotune(sqsg_job *job, int *chosen, double *rate) {
         const uint64_t L = base * f;
         if (f > 1 && (L > saved_it / 2 || static_cast<double>(L) / best_rate > 0.05)) break;
         const double r = timed(std::min<uint64_t>(3 * L, saved_it), L);
-        if (r > best_l) best_l = r, pick = f > 1 ? L : 0;
+        if (r > best_l * (f > 1 ? 1.01 : 1.0)) best_l = r, pick = f > 1 ? L : 0;
       }
       job->tuned_launch = pick;
       best_rate = std::max(best_rate, best_l);
@@ -784,7 +790,7 @@ size_t admission_bytes(const sqsg_job *job, uint64_t slice) {
   size_t rec = 24 + 8 * job->problems.size() + job->stride;
   return job_footprint(job) +
          2 * rec * first_sink_records(job, uint64_t(device_cu_count()) * 8,
-                                      slice ? slice : default_launch_size(job));
+                                      slice ? slice : default_step_size(job));
 }
 
 }  // namespace

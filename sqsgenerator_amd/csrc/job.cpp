@@ -853,10 +853,22 @@ std::shared_ptr<RecordBlock> stage_block(const sqsg_job *job, const uint8_t *sta
   return blk;
 }
 
-uint64_t default_launch_size(const sqsg_job *job) {
+uint64_t default_step_size(const sqsg_job *job) {
   uint64_t work = 0;
   for (const auto &P : job->problems) work += P.pi.size() + 64;
   const uint64_t launch = (uint64_t(1) << 34) / std::max<uint64_t>(work, 1);
+  return std::clamp<uint64_t>(launch, uint64_t(1) << 16, uint64_t(1) << 24);
+}
+
+uint64_t default_launch_size(const sqsg_job *job) {
+  uint64_t work = 0;
+  for (const auto &P : job->problems) work += P.pi.size() + 64;
+  // about 2^34 pair visits per launch for the pair-list kernels; the dense kernel covers a
+  // pair visit about ten times faster and gets 2^36 (4M candidates at 256 sites, ~5 ms), as
+  // each launch ends with its slowest wavefront's last segment and a host merge
+  const int v = job->analyzed ? resolve_kernel(job, job->kernel) : SQSG_KERNEL_SPARSE_WAVE;
+  const bool dense = v == SQSG_KERNEL_DENSE_MFMA || v == SQSG_KERNEL_DENSE_PAIRED;
+  const uint64_t launch = (uint64_t(1) << (dense ? 36 : 34)) / std::max<uint64_t>(work, 1);
   return std::clamp<uint64_t>(launch, uint64_t(1) << 16, uint64_t(1) << 24);
 }
 

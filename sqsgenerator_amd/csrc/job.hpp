@@ -250,7 +250,11 @@ struct RecordSink {
   void drain(sqsg_job *job, uint32_t count, uint32_t nprob,
              const std::vector<std::pair<uint64_t, uint64_t>> &ranges, hipStream_t s);
 };
+// candidates per launch of a single-job run (sqsg_run) without an explicit launch size
 uint64_t default_launch_size(const sqsg_job *job);
+// candidates per persistent step of a batched or served job without an explicit size:
+// about 2^34 pair visits, so that a job's threshold tightens after its first short steps
+uint64_t default_step_size(const sqsg_job *job);
 // the wavefront filter bounds the device records of the job (RunParams::wave_filter)
 bool records_bounded(const sqsg_job *job);
 // initial record sink size of a launch of `launch` candidates over `waves` wavefronts
