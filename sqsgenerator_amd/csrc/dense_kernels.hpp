@@ -247,16 +247,87 @@ __device__ __forceinline__ void shell_terms_table(const PR &pr, uint32_t s, cons
   }
 }
 
+// Search kernels with one one-hot column fewer than species (DevProblem::Kc = K - 1): the
+// last species L = K - 1 has no column, and cc holds the (K-1) x (K-1) counts C[a][b] of the
+// others.  With F[a][b] = C[a][b] + C[b][a] (F[a][a] = 2 C[a][a]) the bonds at the sites of
+// species a add up to D[a] = sum over those sites of their neighbour counts in the shell,
+// which the host has checked to be the same for every candidate (dsum[s][a]).  So
+//     c(a, L) = F[a][L] = D[a] - sum_{b < L} F[a][b]
+//     c(L, L) = (D[L] - sum_{a < L} F[a][L]) / 2
+// exactly, and the terms follow from the same counts as with all K columns, in the same
+// order (optimization.h:85-100).  TABLE: term values from the job's tables (a count is
+// clamped to the table's range, which only a wrong D could leave); else the arithmetic.
+template <int KT, bool TABLE, class PR>
+__device__ __forceinline__ void shell_terms_reduced_k(const PR &pr, uint32_t s,
+                                                      const uint32_t *cc, double &obj) {
+  constexpr int L = KT - 1, NT = KT * (KT + 1) / 2;
+  uint32_t c[NT];
+  uint32_t fl = 0;  // sum over a < L of F[a][L]
+  int t = 0;
+#pragma unroll
+  for (int xi = 0; xi < KT; ++xi)
+#pragma unroll
+    for (int eta = xi; eta < KT; ++eta, ++t) {
+      if (eta < L) {
+        c[t] = cc[xi * L + eta] + (xi == eta ? 0u : cc[eta * L + xi]);
+      } else if (xi < L) {
+        uint32_t f = 0;
+#pragma unroll
+        for (int b = 0; b < L; ++b) f += cc[xi * L + b] + cc[b * L + xi];
+        c[t] = constant_ref(pr.dsum, s * KT + xi) - f;
+        fl += c[t];
+      } else {
+        c[t] = (constant_ref(pr.dsum, s * KT + L) - fl) >> 1;
+      }
+    }
+  if constexpr (TABLE) {
+    const uint32_t at = constant_ref(pr.otab_shell, 2 * s);
+    const uint32_t stride = constant_ref(pr.otab_shell, 2 * s + 1);
+    const double *tab = pr.otab + at;
+    double v[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      const uint32_t cu = c[u] < stride ? c[u] : stride - 1;
+      v[u] = tab[static_cast<uint32_t>(u) * stride + cu];
+    }
+#pragma unroll
+    for (int u = 0; u < NT; ++u) obj += v[u];
+  } else {
+    const TermParam *tp = pr.tparams + s * NT;
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      const auto &e = constant_ref(tp, u);
+      const double v = 1.0 - static_cast<double>(c[u]) * e.pref;
+      obj += e.pw * absolute(v - e.tgt);
+    }
+  }
+}
+
+template <bool TABLE, class PR>
+__device__ __forceinline__ void shell_terms_reduced(const PR &pr, uint32_t s,
+                                                    const uint32_t *cc, double &obj) {
+  switch (pr.K) {  // the host reduces problems of 2-6 species only
+    case 2: return shell_terms_reduced_k<2, TABLE>(pr, s, cc, obj);
+    case 3: return shell_terms_reduced_k<3, TABLE>(pr, s, cc, obj);
+    case 4: return shell_terms_reduced_k<4, TABLE>(pr, s, cc, obj);
+    case 5: return shell_terms_reduced_k<5, TABLE>(pr, s, cc, obj);
+    default: return shell_terms_reduced_k<6, TABLE>(pr, s, cc, obj);
+  }
+}
+
 // Scores the candidates in LDS rows [0, npass * CW) (row c at rows + c * stride, 32 * dT
-// bytes, padding sites 0xff); `cnt` is CW * K * K + 32 words of LDS.  Passes whose bit in
-// `live` are clear are skipped.  With `prune_key` below ~0, a pass stops after the first
-// shell at which no active candidate of it (bit set in `active`) has a partial objective key
-// at or below prune_key: every term is >= 0 (the host checks the pair weights), so partial
-// sums only grow and such candidates can never be kept.  Returns the objective of the
-// lane's own candidate (lane < npass * CW) and, when given, writes its ordered bonds and SRO
-// cube; TABLE: the objective from the job's term tables (no bonds or SRO output).
+// bytes, padding sites 0xff) with Kc one-hot columns per candidate: pr.K, or pr.K - 1 where
+// the host allows the last species to be left out (pr.Kc; search kernels only, no bond or
+// SRO output), which fits 32 / (K - 1) candidates in a tile group instead of 32 / K.  `cnt`
+// is CW * Kc * Kc + 32 words of LDS.  Passes whose bit in `live` are clear are skipped.
+// With `prune_key` below ~0, a pass stops after the first shell at which no active candidate
+// of it (bit set in `active`) has a partial objective key at or below prune_key: every term
+// is >= 0 (the host checks the pair weights), so partial sums only grow and such candidates
+// can never be kept.  Returns the objective of the lane's own candidate (lane < npass * CW)
+// and, when given, writes its ordered bonds and SRO cube; TABLE: the objective from the
+// job's term tables (no bonds or SRO output).
 template <int TT, int G, int D, bool TABLE, class PR>
-__device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
+__device__ __forceinline__ double dense_score(const PR &pr, uint32_t Kc, const uint8_t *rows,
                                               uint32_t stride, uint32_t *cnt,
                                               uint32_t CW, uint32_t npass,
                                               unsigned long long live, int lane, int32_t *bonds,
@@ -264,7 +335,8 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
                                               unsigned long long active = 0,
                                               unsigned long long prune_key = ~0ull) {
   constexpr int P = TT * (TT + 1) / 2;
-  const uint32_t K = pr.K, KK = K * K, T = pr.dT, S = pr.S;
+  const uint32_t K = Kc, KK = K * K, T = pr.dT, S = pr.S;
+  const bool reduced = Kc != pr.K;
   const uint32_t g = CW / G;
   const uint32_t h = static_cast<uint32_t>(lane) >> 5, n = static_cast<uint32_t>(lane) & 31u;
   const uint32_t cl = n / K, b = n - cl * K;
@@ -276,7 +348,7 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   const uint32_t toff = static_cast<uint32_t>(lane) * 16u;
   const uint32_t my_pass = static_cast<uint32_t>(lane) / CW;
   const uint32_t my_cc = static_cast<uint32_t>(lane) - my_pass * CW;  // candidate in its pass
-  const uint32_t cube = pr.S * KK;
+  const uint32_t cube = pr.S * pr.K * pr.K;
   // a spare word past the counts (lanes l and l + 32 share one: its value is never read)
   const uint32_t trash = CW * KK + (static_cast<uint32_t>(lane) & 31u);
   // accumulator registers of GEMM2 on this lane's diagonal block
@@ -286,7 +358,9 @@ __device__ __forceinline__ double dense_score(const PR &pr, const uint8_t *rows,
   // reference order
   auto terms_of = [&](uint32_t opass, uint32_t os) {
     if (my_pass == opass) {
-      if constexpr (TABLE) {
+      if (reduced) {
+        shell_terms_reduced<TABLE>(pr, os, cnt + my_cc * KK, obj);
+      } else if constexpr (TABLE) {
         shell_terms_table(pr, os, cnt + my_cc * KK, obj);
       } else {
         const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
@@ -431,8 +505,9 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
       prune_key = g0 < l0 ? g0 : l0;
     }
     const double tot =
-        dense_score<TT, G, D, kDenseTermTables<TT>>(pr, rows, stride, cnt, CW, npass, live,
-                                                    lane, nullptr, nullptr, 0u, am, prune_key);
+        dense_score<TT, G, D, kDenseTermTables<TT>>(pr, pr.Kc, rows, stride, cnt, CW, npass,
+                                                    live, lane, nullptr, nullptr, 0u, am,
+                                                    prune_key);
     scored += static_cast<unsigned long long>(__popcll(am));
     // a candidate above the threshold, or above the filter's L-th level when its turn
     // comes, cannot be inserted
@@ -488,8 +563,8 @@ __device__ __forceinline__ void dense_segment_split(const RP &p, uint64_t sbeg, 
   double tot = 0.0;
   for (uint32_t q = 0; q < nprob; ++q) {
     const double o = dense_score<TT, G, D, kDenseTermTables<TT>>(
-        p.prob[q], wl + p.off_rows[q], stride, cnt, CW, npass, live, lane, nullptr, nullptr, 0u,
-        am, ~0ull);
+        p.prob[q], p.prob[q].Kc, wl + p.off_rows[q], stride, cnt, CW, npass, live, lane, nullptr,
+        nullptr, 0u, am, ~0ull);
     subs[q * 64 + static_cast<uint32_t>(lane)] = o;
     tot += o;
   }
@@ -625,7 +700,7 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * GEN;
   const uint32_t out_rows = static_cast<uint32_t>(p.B - b0 < GEN ? p.B - b0 : GEN);
   const double obj = dense_score<TT, G, D, false>(
-      pr, rows, p.dense_stride, cnt, CW, npass, live, lane,
+      pr, pr.K, rows, p.dense_stride, cnt, CW, npass, live, lane,
       p.bonds ? p.bonds + b0 * nb : nullptr, p.sro ? p.sro + b0 * nb : nullptr, out_rows);
   if (active) p.objective[b] = obj;
 }

@@ -39,6 +39,12 @@ struct DevProblem {
   // otab[otab_shell[2s]], term t of it at + t * otab_shell[2s + 1] (P_s + 1 counts)
   const double *otab;
   const uint32_t *otab_shell;
+  // one-hot columns per candidate of the dense search kernels: K, or K - 1 when the bonds of
+  // the last species follow from the others' (every shuffle window's sites have equal
+  // neighbour counts in each shell); then dsum[s][a] is the sum of the shell-s neighbour
+  // counts of the sites holding species a, the same for every candidate
+  const uint32_t *dsum;
+  uint32_t Kc;
   uint32_t dT;                // 32-site tiles
   uint32_t dtiles_n;
   // lane-per-candidate layout: per shell s, groups [i | count << 16] followed by `count`

@@ -222,6 +222,31 @@ void Problem::build_lane(uint32_t field_bits) {
   upload(d_species0F, sF, 16);
 }
 
+// The last species needs no one-hot column in the dense search kernels when the
+// neighbour-count sum of each species' sites is the same for every candidate.  Candidates
+// permute species within the shuffle windows, so it is when every window's sites have equal
+// neighbour counts in each shell (sites outside the windows keep their species); the sums
+// are then those of the sorted configuration.  Problems of 2-6 species.
+void Problem::reduce_columns() {
+  Kc = K;
+  dsum.clear();
+  if (K < 2 || K > 6) return;
+  std::vector<uint32_t> deg(size_t(S) * N, 0);
+  for (size_t q = 0; q < pi.size(); ++q) {
+    ++deg[size_t(ps[q]) * N + pi[q]];
+    ++deg[size_t(ps[q]) * N + pj[q]];
+  }
+  for (uint32_t s = 0; s < S; ++s)
+    for (size_t w = 0; w + 1 < bounds.size(); w += 2)
+      for (uint32_t i = bounds[w]; i < bounds[w + 1]; ++i)
+        if (deg[size_t(s) * N + i] != deg[size_t(s) * N + bounds[w]]) return;
+  dsum.assign(size_t(S) * K, 0);
+  for (uint32_t s = 0; s < S; ++s)
+    for (uint32_t i = 0; i < N; ++i)
+      dsum[size_t(s) * K + species_packed[i]] += deg[size_t(s) * N + i];
+  Kc = K - 1;
+}
+
 // Tiles in the A-operand layout of v_mfma_i32_32x32x32_i8: lane l, byte q of tile (s, I, J)
 // is U_s[32I + perm(l % 32)][32J + 16(l / 32) + q] with perm(m) = 16((m / 4) % 2) +
 // 4(m / 8) + m % 4 (the accumulator row order of the 32 x 32 result).  Layout invariant
@@ -233,6 +258,7 @@ void Problem::build_lane(uint32_t field_bits) {
 // one shell past the end stays inside the stream: dtiles_n = (S + 1) TT (TT + 1) / 2.
 void Problem::build_dense(uint32_t TT) {
   if (d_dtiles.size()) return;
+  if (Kc < K) upload(d_dsum, dsum);
   dT = (N + 31) / 32;
   const uint32_t n32 = 32 * dT;
   if (TT < dT) throw std::logic_error("tile bound below the problem's tile count");
@@ -330,6 +356,7 @@ void analyze_job(sqsg_job *job) {
       if (c >= 65536) job->lane_ok = false;  // 16-bit histogram fields
   }
   for (auto &P : job->problems) {
+    P.reduce_columns();
     std::vector<uint32_t> deg(size_t(P.S) * P.N, 0);
     P.max_upper = 0;
     for (size_t q = 0; q < P.pi.size(); ++q)
@@ -383,6 +410,8 @@ DevProblem Problem::dev() const {
   d.species0D = d_species0D.as<const uint8_t>();
   d.otab = d_otab.as<const double>();
   d.otab_shell = d_otab_shell.as<const uint32_t>();
+  d.dsum = d_dsum.as<const uint32_t>();
+  d.Kc = Kc ? Kc : K;
   d.dT = dT;
   d.dtiles_n = dtiles_n;
   d.N = N, d.K = K, d.S = S;
@@ -550,6 +579,9 @@ std::string kernel_shape(sqsg_job *job) {
       std::string s = "dense<" +
                       (alt ? dense_alt_shape_string(TT, alt) : dense_shape_string(TT, false)) +
                       ">";
+      s += " columns ";
+      for (size_t q = 0; q < job->problems.size(); ++q)
+        s += (q ? "+" : "") + std::to_string(job->problems[q].Kc);
       if (job->dense_passes) s += " passes " + std::to_string(job->dense_passes);
       return s;
     }
@@ -668,16 +700,21 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
   if (variant == SQSG_KERNEL_DENSE_MFMA) {
     // per problem rows of gen_group candidates (npass passes of dense_cw), then one shell's
     // counts; split jobs: the problems share the tile bound, the row stride and the pass
-    // width of the largest species count, and each lane's objectives per problem follow
+    // width of the largest count of one-hot columns, and each lane's objectives per problem
+    // follow.  The search kernels take a problem's Kc columns per candidate, explicit
+    // scoring its K.
+    ensure_dense(job);
+    for (uint32_t q = 0; q < p.nprob; ++q) p.prob[q] = job->problems[q].dev();
     const uint32_t nprob = p.nprob;
-    uint32_t T = 0, Kmax = 0;
+    uint32_t T = 0, Kmax = 0, Cmax = 0;
     for (const auto &P : job->problems) {
       T = std::max(T, (P.N + 31) / 32);
       Kmax = std::max(Kmax, P.K);
+      Cmax = std::max(Cmax, search ? P.Kc : P.K);
     }
     const uint32_t TT = dense_tile_bound(T), alt = search ? search_alt(job) : 0u;
     const uint32_t G = alt ? dense_alt_groups(TT, alt) : dense_groups(TT);
-    const uint32_t g = std::min(32 / Kmax, 64 / G);
+    const uint32_t g = std::min(32 / Cmax, 64 / G);
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
     // one shell's counts of a pass and 32 spare words; then the tie filter levels

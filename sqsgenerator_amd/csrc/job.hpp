@@ -94,6 +94,12 @@ struct Problem {
   // of the bond count c = c_ab + c_ba; per shell its first value and its term stride
   size_t otab_n = 0;
   DevBuf d_otab, d_otab_shell;
+  // one-hot columns of the dense search kernels (K, or K - 1: DevProblem::Kc) and the
+  // per-shell, per-species neighbour-count sums that let the last species go without one
+  uint32_t Kc = 0;
+  std::vector<uint32_t> dsum;
+  DevBuf d_dsum;
+  void reduce_columns();
   void build_dense(uint32_t tile_bound);  // the kernel's tile bound (the job's largest)
   DevProblem dev() const;
   void upload_device();

@@ -80,6 +80,10 @@ MC_CASES = {
     "fcc256": (lambda: W.quinary_256(seed=12), 20001, 6000, (2, 3)),
     "rocksalt_split": (lambda: W.rocksalt_split_216(seed=13), 5001, 1500, (1, 2, 3)),
     "split_uneven": (_split_uneven, 5001, 1500, (1, 2, 3)),
+    # the dense kernel without the last species' column, windows of different neighbour
+    # counts and fixed sites; and with every column, one window of mixed counts
+    "perovskite_bx": (lambda: W.perovskite_bx(seed=22), 20001, 6000, (1, 2, 3)),
+    "perovskite_mixed": (lambda: W.perovskite_bx(seed=23, mixed=True), 20001, 6000, (1, 2, 3)),
 }
 
 
@@ -167,9 +171,11 @@ def test_callback_exception_propagates(gpu):
 
 
 def test_batch_packs_match_oracle(gpu):
-    """the batched persistent form over uneven jobs (configs (a)-(d) at 1e5 candidates each,
-    keep 2, one of them with max_results_per_objective) against the oracle's driver"""
-    makers = [W.ternary_108, W.quaternary_bcc_128, W.hcp_128, W.rocksalt_split_216]
+    """the batched persistent form over uneven jobs (configs (a)-(d) and a perovskite cell
+    of another tile bound, at 1e5 candidates each, keep 2, one of them with
+    max_results_per_objective) against the oracle's driver"""
+    makers = [W.ternary_108, W.quaternary_bcc_128, W.hcp_128, W.rocksalt_split_216,
+              W.perovskite_bx]
     cfgs, jobs = [], []
     for k, make in enumerate(makers):
         d = make(iterations=100_001)

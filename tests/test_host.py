@@ -190,7 +190,7 @@ def test_kernel_shape_follows_the_applicability_rules():
     jobs of at most 256 sites; the lane kernel is sized by K"""
     from sqsgenerator_amd.core import Job
     job = Job(parse_config(W.quinary_256(iterations=1000)))
-    assert job.kernel_shape() == "dense<8,2,2,3>"
+    assert job.kernel_shape() == "dense<8,2,2,3> columns 4"
     job.set_kernel(4)
     assert job.kernel_shape() == "paired<8,2,9>"
     job.set_kernel(2)
@@ -199,11 +199,30 @@ def test_kernel_shape_follows_the_applicability_rules():
     assert job.kernel_shape() == "wave"
     job.close()
     big = Job(parse_config(W.quinary_864(iterations=1000)))
-    assert big.kernel_shape() == "dense<27,1,2,9>"
+    assert big.kernel_shape() == "dense<27,1,2,9> columns 4"
     big.set_kernel(4)
     with pytest.raises(ValueError):
         big.kernel_shape()
     big.close()
+
+
+@pytest.mark.parametrize("make,K,shape", [
+    (lambda: W.quinary_256(iterations=1000), 5, "dense<8,2,2,3> columns 4"),
+    (lambda: W.ternary_108(iterations=1000), 3, "dense<4,2,2,5> columns 2"),
+    (lambda: W.rocksalt_split_216(iterations=1000), 2, "dense<4,2,2,5> columns 1+1"),
+    # B and X windows with 6 and 2 neighbours at a/2, the A sites (Sr) fixed
+    (lambda: W.perovskite_bx(), 5, "dense<6,2,2,3> columns 4"),
+    # one window over sites with different neighbour counts: every species keeps a column
+    (lambda: W.perovskite_bx(mixed=True), 4, "dense<6,2,2,3> columns 4"),
+])
+def test_dense_columns_per_candidate(make, K, shape):
+    """the dense search kernels leave the last species' one-hot column out wherever every
+    shuffle window's sites have equal neighbour counts in each shell"""
+    from sqsgenerator_amd.core import Job
+    job = Job(parse_config(make()))
+    assert len(set(job.problem(0)["species_packed"].tolist())) == K
+    assert job.kernel_shape() == shape
+    job.close()
 
 
 @pytest.mark.parametrize("sites,cell,bound", [(32, (2, 2, 2), 1), (96, (2, 3, 4), 4),

@@ -102,6 +102,31 @@ def rocksalt_split_216(iterations=10_000_000, seed=1004):
                 iterations=iterations, seed=[seed, seed + 1], thread_config=1)
 
 
+PEROVSKITE_BASIS = [[0.0, 0.0, 0.0], [0.5, 0.5, 0.5], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5],
+                    [0.0, 0.5, 0.5]]
+
+
+def perovskite_bx(seed=21, mixed=False, iterations=100_000):
+    """cubic ABX3 perovskite 3x3x3 (a = 3.9; A Sr, B Ti, X O; 135 sites), the A sites left
+    out.  Sublattices B (Ti/Zr) and X (O/F), whose sites have different neighbour counts
+    (6 and 2 at a/2); mixed: one sublattice over the B and X sites together (Ti/Zr/O), where
+    the counts differ inside the shuffle window"""
+    a = 3.9
+    d = dict(structure=dict(lattice=np.diag([a, a, a]).tolist(), coords=PEROVSKITE_BASIS,
+                            species=["Sr", "Ti", "O", "O", "O"], supercell=[3, 3, 3]),
+             shell_weights={"1": 1.0, "2": 0.5, "3": 1.0 / 3.0},
+             iterations=iterations, seed=seed, thread_config=1)
+    if mixed:
+        from sqsgenerator_amd import parse_config
+        st = parse_config(dict(d, composition={"sites": "Ti", "Ti": 27})).structure()
+        bx = [i for i, z in enumerate(st.species) if z in (8, 22)]
+        d["composition"] = {"sites": bx, "Ti": 36, "Zr": 36, "O": 36}
+    else:
+        d["composition"] = [{"sites": "Ti", "Ti": 14, "Zr": 13},
+                            {"sites": "O", "O": 41, "F": 40}]
+    return d
+
+
 def reference_test_16(**kw):
     """python/tests/test_core.py:27-42: 16 sites, Ti8Al8"""
     d = dict(iteration_mode="systematic",
