@@ -319,7 +319,7 @@ __device__ __forceinline__ void shell_terms_reduced(const PR &pr, uint32_t s,
 // bytes, padding sites 0xff) with Kc one-hot columns per candidate: pr.K, or pr.K - 1 where
 // the host allows the last species to be left out (pr.Kc; search kernels only, no bond or
 // SRO output), which fits 32 / (K - 1) candidates in a tile group instead of 32 / K.  `cnt`
-// is CW * Kc * Kc + 32 words of LDS.  Passes whose bit in `live` are clear are skipped.
+// is CW * (Kc * Kc | 1) + 32 words of LDS.  Passes whose bit in `live` are clear are skipped.
 // With `prune_key` below ~0, a pass stops after the first shell at which no active candidate
 // of it (bit set in `active`) has a partial objective key at or below prune_key: every term
 // is >= 0 (the host checks the pair weights), so partial sums only grow and such candidates
@@ -349,8 +349,13 @@ __device__ __forceinline__ double dense_score(const PR &pr, uint32_t Kc, const u
   const uint32_t my_pass = static_cast<uint32_t>(lane) / CW;
   const uint32_t my_cc = static_cast<uint32_t>(lane) - my_pass * CW;  // candidate in its pass
   const uint32_t cube = pr.S * pr.K * pr.K;
+  // a candidate's counts take an odd number of words, so the lanes that read different
+  // candidates' counts together hit different LDS banks
+  const uint32_t cs = KK | 1u;
   // a spare word past the counts (lanes l and l + 32 share one: its value is never read)
-  const uint32_t trash = CW * KK + (static_cast<uint32_t>(lane) & 31u);
+  const uint32_t trash = CW * cs + (static_cast<uint32_t>(lane) & 31u);
+  // count word of accumulator row m of tile group gi: candidate gi * g + cl, row m - cl * K
+  const uint32_t cbase = cl * cs - cl * K * K + b;
   // accumulator registers of GEMM2 on this lane's diagonal block
   const uint32_t keep = diag_registers(cl * K, K, h, col_ok);
   double obj = 0.0;
@@ -359,13 +364,13 @@ __device__ __forceinline__ double dense_score(const PR &pr, uint32_t Kc, const u
   auto terms_of = [&](uint32_t opass, uint32_t os) {
     if (my_pass == opass) {
       if (reduced) {
-        shell_terms_reduced<TABLE>(pr, os, cnt + my_cc * KK, obj);
+        shell_terms_reduced<TABLE>(pr, os, cnt + my_cc * cs, obj);
       } else if constexpr (TABLE) {
-        shell_terms_table(pr, os, cnt + my_cc * KK, obj);
+        shell_terms_table(pr, os, cnt + my_cc * cs, obj);
       } else {
         const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
         const bool out = cr < out_rows;
-        shell_terms(pr, os, cnt + my_cc * KK, obj, out && bonds ? bonds + cr * cube : nullptr,
+        shell_terms(pr, os, cnt + my_cc * cs, obj, out && bonds ? bonds + cr * cube : nullptr,
                     out && sro ? sro + cr * cube : nullptr);
       }
     }
@@ -431,7 +436,7 @@ __device__ __forceinline__ double dense_score(const PR &pr, uint32_t Kc, const u
 #pragma unroll
         for (uint32_t r = 0; r < 16; ++r) {
           const uint32_t m = 8 * (r >> 2) + 4 * h + (r & 3);
-          const uint32_t at = ((keep >> r) & 1u) ? gi * g * KK + K * m + b : trash;
+          const uint32_t at = ((keep >> r) & 1u) ? gi * g * cs + cbase + K * m : trash;
           cnt[at] = static_cast<uint32_t>(C[gi][r]);
         }
       wave_sync();

@@ -718,7 +718,7 @@ RunParams make_run_params(sqsg_job *job, int variant, uint32_t W, uint32_t *bloc
     p.dense_cw = G * g;
     p.dense_stride = 32 * T + 16;
     // one shell's counts of a pass and 32 spare words; then the tie filter levels
-    const uint32_t cnt_bytes = align(p.dense_cw * Kmax * Kmax * 4 + 32 * 4);
+    const uint32_t cnt_bytes = align(p.dense_cw * (Kmax * Kmax | 1u) * 4 + 32 * 4);
     const uint32_t lf_bytes = align(static_cast<uint32_t>(sizeof(LevelFilter)));
     const uint32_t obj_bytes = nprob > 1 ? align(nprob * 64 * 8) : 0;
     auto rows_bytes = [&](uint32_t npass) {
