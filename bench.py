@@ -39,6 +39,16 @@ def dense_ops_per_candidate(N, K, S):
     return S * (2 * N * N * K + 2 * N * K * K)
 
 
+def executed_ops_per_candidate(shape, S):
+    """int8 ops the dense kernel of launch shape `shape` ("dense<TT,G,W,D> columns C")
+    executes per candidate: per shell and tile group, TT (TT + 1) / 2 GEMM1 and TT GEMM2
+    32x32x32 products for 32 // C candidates"""
+    head, _, cols = shape.partition(" columns ")
+    TT = int(head[head.index("<") + 1:].split(",")[0])
+    C = int(cols.split()[0].split("+")[0])
+    return S * (TT * (TT + 1) // 2 + TT) * 2 * 32 ** 3 // (32 // C)
+
+
 def measured_traffic(candidates_per_launch):
     """HBM bytes per launch from the committed rocprofv3 counter summary (FETCH_SIZE and
     WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), if it matches this
@@ -500,7 +510,7 @@ def main() -> int:
     # run right before the timed ones (the device is at its steady clock when timing starts)
     jobs = batch_jobs(args.steps)
     info = jobs[0].problem(0)
-    shape = jobs[0].kernel_shape()  # e.g. dense<8,2,2,3>
+    shape = jobs[0].kernel_shape()  # e.g. dense<8,2,2,3> columns 4
     kernel_name = ("batch_dense_kernel" + shape[len("dense"):] if shape.startswith("dense")
                    else shape)
     warm = batch_jobs(args.warmup) if args.warmup else []
@@ -563,10 +573,14 @@ def main() -> int:
                      "traffic": measured_traffic(per_launch),
                      "kernel": kernel_name, "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_ops_per_candidate": ops_per_cand,
+                     "executed_ops_per_candidate": (executed_ops_per_candidate(shape, S)
+                                                    if " columns " in shape else None),
                      "candidates_per_launch": per_launch,
                      "note": "int8 ops (2 per multiply-accumulate) of the dense one-hot "
-                             "contraction S(2N^2K + 2NK^2) per candidate; launch time from HIP "
-                             "events around the graph launch on the engine's stream"},
+                             "contraction S(2N^2K + 2NK^2) per candidate; the kernel executes "
+                             "fewer (upper tiles, the last species without a column); launch "
+                             "time from HIP events around the graph launch on the engine's "
+                             "stream"},
     }
     if not args.no_large_cells:
         out["large_cell_scoring"] = large_cell_scoring(args.large_cell_launches)
