@@ -49,6 +49,60 @@ __device__ __forceinline__ int onehot4(uint32_t w, uint32_t bb) {
   return static_cast<int>((~t >> 7) & 0x01010101u);
 }
 
+// One-hot tiles of a pass: X[gi][J] of lane (h, n) holds, for its column's candidate
+// first + gi * g + cl (row at rows + c * stride) and species b (bb = b replicated), bytes
+// [species(32J + 16h + q) == b], q = 0..15; zero for tiles J >= T and for columns past the
+// group's candidates (col_ok false).  BATCHED issues every row read before the first
+// conversion (tiles past T read tile 0 and are zeroed): one LDS round trip instead of one per
+// tile, where the registers allow it.
+template <int TT, int G, bool BATCHED>
+__device__ __forceinline__ void onehot_tiles(v4i (&X)[G][TT], const uint8_t *rows,
+                                             uint32_t first, uint32_t g, uint32_t cl,
+                                             bool col_ok, uint32_t stride, uint32_t h,
+                                             uint32_t T, uint32_t bb) {
+  if constexpr (BATCHED) {
+    uint4 raw[G][TT];
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      const uint8_t *row = rows + (col_ok ? first + gi * g + cl : 0u) * stride + 16 * h;
+#pragma unroll
+      for (int J = 0; J < TT; ++J)
+        raw[gi][J] =
+            *reinterpret_cast<const uint4 *>(row + 32 * (static_cast<uint32_t>(J) < T ? J : 0));
+    }
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+      for (int J = 0; J < TT; ++J) {
+        const bool on = static_cast<uint32_t>(J) < T && col_ok;
+        const uint4 w = raw[gi][J];
+        v4i x;
+        x[0] = on ? onehot4(w.x, bb) : 0;
+        x[1] = on ? onehot4(w.y, bb) : 0;
+        x[2] = on ? onehot4(w.z, bb) : 0;
+        x[3] = on ? onehot4(w.w, bb) : 0;
+        X[gi][J] = x;
+      }
+  } else {
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      const uint8_t *row = rows + (col_ok ? first + gi * g + cl : 0u) * stride + 16 * h;
+#pragma unroll
+      for (int J = 0; J < TT; ++J) {
+        v4i x = {0, 0, 0, 0};
+        if (static_cast<uint32_t>(J) < T && col_ok) {
+          const uint4 w = *reinterpret_cast<const uint4 *>(row + 32 * J);
+          x[0] = onehot4(w.x, bb);
+          x[1] = onehot4(w.y, bb);
+          x[2] = onehot4(w.z, bb);
+          x[3] = onehot4(w.w, bb);
+        }
+        X[gi][J] = x;
+      }
+    }
+  }
+}
+
 // GEMM1 accumulator (16 counts <= 127 of one column) -> GEMM2 B operand bytes.  Per word
 // three byte permutes (v_perm_b32: selector bytes 0-3 take bytes of the second operand, 4-7
 // of the first, 0x0c gives zero): [y0, y1, 0, 0] and [0, 0, y2, y3], then their bytes
@@ -377,24 +431,10 @@ __device__ __forceinline__ double dense_score(const PR &pr, uint32_t Kc, const u
   };
   for (uint32_t pass = 0; pass < npass; ++pass) {
     if (!((live >> pass) & 1ull)) continue;
+    // batched row reads up to the 8-tile bound (+0.8% at 256 sites); the larger bounds'
+    // registers do not hold them
     v4i X[G][TT];
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi) {
-      const uint32_t c = pass * CW + gi * g + cl;
-      const uint8_t *row = rows + (col_ok ? c : 0u) * stride + 16 * h;
-#pragma unroll
-      for (int J = 0; J < TT; ++J) {
-        v4i x = {0, 0, 0, 0};
-        if (static_cast<uint32_t>(J) < T && col_ok) {
-          const uint4 w = *reinterpret_cast<const uint4 *>(row + 32 * J);
-          x[0] = onehot4(w.x, bb);
-          x[1] = onehot4(w.y, bb);
-          x[2] = onehot4(w.z, bb);
-          x[3] = onehot4(w.w, bb);
-        }
-        X[gi][J] = x;
-      }
-    }
+    onehot_tiles<TT, G, (TT <= 8)>(X, rows, pass * CW, g, cl, col_ok, stride, h, T, bb);
     // ring slot q % D holds grid position q (row-major over I <= J < TT) of the shell.
     // Grid positions are stepped incrementally so that every index folds to a constant
     // once the loops are unrolled.
@@ -808,53 +848,10 @@ __global__ __launch_bounds__(64 * kShellWaves, LT ? 2 : 1) void score_shell_kern
   const uint32_t cl = n / K, bcol = n - cl * K;
   const bool col_ok = cl < g;
   const uint32_t bb = bcol * 0x01010101u;
+  // the long rows of the large bounds: every row read is issued before the first one-hot
+  // conversion (-3% at 864 sites against one conditional read at a time)
   v4i X[G][TT];
-  if constexpr (LT) {
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi) {
-      const uint32_t c = gi * g + cl;
-      const uint8_t *row = rows + (col_ok ? c : 0u) * stride + 16 * h;
-#pragma unroll
-      for (int J = 0; J < TT; ++J) {
-        v4i x = {0, 0, 0, 0};
-        if (static_cast<uint32_t>(J) < T && col_ok) {
-          const uint4 w = *reinterpret_cast<const uint4 *>(row + 32 * J);
-          x[0] = onehot4(w.x, bb);
-          x[1] = onehot4(w.y, bb);
-          x[2] = onehot4(w.z, bb);
-          x[3] = onehot4(w.w, bb);
-        }
-        X[gi][J] = x;
-      }
-    }
-  } else {
-    // the long rows of the large bounds: every row read is issued before the first one-hot
-    // conversion (tiles past T read tile 0 and are zeroed), -3% at 864 sites against one
-    // conditional read at a time
-    uint4 raw[G][TT];
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi) {
-      const uint32_t c = gi * g + cl;
-      const uint8_t *row = rows + (col_ok ? c : 0u) * stride + 16 * h;
-#pragma unroll
-      for (int J = 0; J < TT; ++J)
-        raw[gi][J] =
-            *reinterpret_cast<const uint4 *>(row + 32 * (static_cast<uint32_t>(J) < T ? J : 0));
-    }
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi)
-#pragma unroll
-      for (int J = 0; J < TT; ++J) {
-        const bool on = static_cast<uint32_t>(J) < T && col_ok;
-        const uint4 w = raw[gi][J];
-        v4i x;
-        x[0] = on ? onehot4(w.x, bb) : 0;
-        x[1] = on ? onehot4(w.y, bb) : 0;
-        x[2] = on ? onehot4(w.z, bb) : 0;
-        x[3] = on ? onehot4(w.w, bb) : 0;
-        X[gi][J] = x;
-      }
-  }
+  onehot_tiles<TT, G, !LT>(X, rows, 0u, g, cl, col_ok, stride, h, T, bb);
   v16i C[G], Y[G];
 #pragma unroll
   for (int gi = 0; gi < G; ++gi) C[gi] = v16i{};
@@ -1052,24 +1049,8 @@ __device__ __forceinline__ void ws_mfma_tick(const PR &pr, const uint8_t *rows, 
       static_cast<int>(pr.dtiles_n * 1024u), 0x00020000);
   const uint32_t toff = static_cast<uint32_t>(lane) * 16u;
   if (s == 0) {
-    const uint32_t bb = b * 0x01010101u;
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi) {
-      const uint32_t c = pass * CW + gi * g + cl;
-      const uint8_t *row = rows + (col_ok ? c : 0u) * stride + 16 * h;
-#pragma unroll
-      for (int J = 0; J < TT; ++J) {
-        v4i x = {0, 0, 0, 0};
-        if (static_cast<uint32_t>(J) < T && col_ok) {
-          const uint4 w = *reinterpret_cast<const uint4 *>(row + 32 * J);
-          x[0] = onehot4(w.x, bb);
-          x[1] = onehot4(w.y, bb);
-          x[2] = onehot4(w.z, bb);
-          x[3] = onehot4(w.w, bb);
-        }
-        X[gi][J] = x;
-      }
-    }
+    onehot_tiles<TT, G, false>(X, rows, pass * CW, g, cl, col_ok, stride, h, T,
+                               b * 0x01010101u);
 #pragma unroll
     for (int q = 0; q < D; ++q) ring[q] = tile_load(rs, toff, tile_offset<TT>(0, q));
   }
