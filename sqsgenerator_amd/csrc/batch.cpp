@@ -511,6 +511,7 @@ class BatchEngine {
     bp.wave_lds = wave_lds;
     bp.claims = g.d_claims.as<unsigned int>();
     bp.max_items = kGroupItems;
+    bp.stagger_from = blocks / 2;
     g.blocks = blocks;
     SQSG_HIP(g.variant == SQSG_KERNEL_DENSE_MFMA
                  ? (g.alt ? prepare_batch_dense_alt(g.shape, g.alt, block_lds)

@@ -639,12 +639,14 @@ __device__ __forceinline__ void dense_segment_split(const RP &p, uint64_t sbeg, 
 
 // The two wavefronts of a SIMD start together and, doing the same work at the same pace, stay
 // in step: both shuffle candidates (vector work only) at the same time, then both contend
-// for the matrix core.  The second half of the grid, which the dispatcher places on the
-// SIMDs beside the first half, starts about 16k cycles later, so that one wavefront's
-// vector phases tend to fall in the other's MFMA phases (+2% at 256 sites; at worst a
-// fraction of a percent of a launch where the placement differs).
-__device__ __forceinline__ void stagger_start() {
-  if (blockIdx.x >= gridDim.x / 2) {
+// for the matrix core.  The second half of the grid (blocks from `from`, half the grid as
+// the host passes it), which the dispatcher places on the SIMDs beside the first half,
+// starts about 16k cycles later, so that one wavefront's vector phases tend to fall in the
+// other's MFMA phases (+1-2% at 256 sites; at worst a fraction of a percent of a launch
+// where the placement differs).  The grid size comes from the parameters: reading it from
+// the dispatch packet holds two more scalar registers through the kernel, which spilled.
+__device__ __forceinline__ void stagger_start(uint32_t from) {
+  if (blockIdx.x >= from) {
     __builtin_amdgcn_s_sleep(127);
     __builtin_amdgcn_s_sleep(127);
   }
@@ -654,7 +656,7 @@ template <int TT, int G, int W, int D, bool SPLIT>
 __global__ __launch_bounds__(64, W) void run_dense_kernel(RunParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
-  stagger_start();
+  stagger_start(p.stagger_from);
   const uint64_t per_seg = static_cast<uint64_t>(p.gen_group) * p.rounds;
   const uint64_t total = p.cand_end - p.cand_begin;
   const uint64_t nseg = (total + per_seg - 1) / per_seg;
@@ -685,7 +687,7 @@ template <int TT, int G, int W, int D, bool SPLIT>
 __global__ __launch_bounds__(64, W) void batch_dense_kernel(BatchParams bp) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
-  stagger_start();
+  stagger_start(bp.stagger_from);
   const uint32_t num_items = *bp.num_items;
   uint32_t last_job = 0xffffffffu, it = 0, job = 0;
   uint64_t cbeg = 0, cend = 0;

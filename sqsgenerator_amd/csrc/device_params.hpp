@@ -114,6 +114,7 @@ struct RunParams {
   uint32_t rec_stride;
   unsigned long long *scored;
   unsigned int *next_seg;      // dense single-job kernel: segment counter (zero at launch)
+  uint32_t stagger_from;       // dense search kernels: blocks from this index start later
   // persistent form: host memory (pinned, device-visible) that receives the record and
   // scored counts and the first stage_cap records when the step ends, or null
   uint8_t *stage;
@@ -143,6 +144,7 @@ struct BatchParams {
   uint32_t wave_lds;           // LDS of one wavefront: the largest job's layout
   unsigned int *claims;        // per-item chunk counters (zeroed by the reset kernel)
   uint32_t max_items;          // rows of the item table
+  uint32_t stagger_from;       // dense search kernels: blocks from this index start later
 };
 
 struct ScoreParams {

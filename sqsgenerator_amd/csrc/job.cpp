@@ -961,6 +961,7 @@ void run_job(sqsg_job *job, sqsg_progress_fn cb, void *user, const FixedScratch 
   p.rec_count = reinterpret_cast<unsigned int *>(ctl + 8);
   p.scored = reinterpret_cast<unsigned long long *>(ctl + 16);
   p.next_seg = reinterpret_cast<unsigned int *>(ctl + 24);
+  p.stagger_from = blocks / 2;
   const auto t_loop = clock::now();
   job->stats.timing_chunk_setup_ns =
       std::chrono::duration_cast<std::chrono::nanoseconds>(t_loop - t_start).count();
