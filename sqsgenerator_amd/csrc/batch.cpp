@@ -144,7 +144,7 @@ size_t job_footprint(const sqsg_job *job) {
   size_t b = 0;
   for (const auto &P : job->problems)
     b += P.pi.size() * 4 + P.lane_stream.size() * 4 + P.pref.size() * 24 + 2 * P.N + 1024 +
-         size_t(P.dtiles_n) * 1024 + 32 * P.dT + P.otab_n * 8;
+         size_t(P.dtiles_n) * 1024 + 32 * P.dT;
   return b;
 }
 

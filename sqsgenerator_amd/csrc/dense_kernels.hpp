@@ -249,58 +249,6 @@ __device__ __forceinline__ void shell_terms(const PR &pr, uint32_t s, const uint
   }
 }
 
-// Search kernels of tile bounds up to 8 read the objective terms from value tables: +1%
-// at 256 sites, while at 500-1024 sites the tables compete with the tile stream for L2
-// (-4% measured at 1024 sites), so the larger bounds keep the arithmetic.
-template <int TT> constexpr bool kDenseTermTables = TT <= 8;
-
-// The same terms from the job's value tables (search kernels, no SRO or bond output): term
-// t's value is otab[shell start + t * stride + c] for its bond count c = c_ab + c_ba, the
-// host having applied the operations above to every count; the loads of a shell are
-// independent and only the additions keep the reference order.
-template <int KT, class PR>
-__device__ __forceinline__ void shell_terms_table_k(const PR &pr, uint32_t s, const uint32_t *cc,
-                                                    double &obj, uint32_t Kr) {
-  const uint32_t K = KT > 0 ? static_cast<uint32_t>(KT) : Kr;
-  const uint32_t at = constant_ref(pr.otab_shell, 2 * s);
-  const uint32_t stride = constant_ref(pr.otab_shell, 2 * s + 1);
-  const double *tab = pr.otab + at;
-  if constexpr (KT > 0) {
-    constexpr int NT = KT * (KT + 1) / 2;
-    double v[NT];
-    int t = 0;
-#pragma unroll
-    for (int xi = 0; xi < KT; ++xi)
-#pragma unroll
-      for (int eta = xi; eta < KT; ++eta, ++t) {
-        const uint32_t c = cc[xi * KT + eta] + (xi == eta ? 0u : cc[eta * KT + xi]);
-        v[t] = tab[static_cast<uint32_t>(t) * stride + c];
-      }
-#pragma unroll
-    for (int u = 0; u < NT; ++u) obj += v[u];
-  } else {
-    uint32_t t = 0;
-    for (uint32_t xi = 0; xi < K; ++xi)
-      for (uint32_t eta = xi; eta < K; ++eta, ++t) {
-        const uint32_t c = cc[xi * K + eta] + (xi == eta ? 0u : cc[eta * K + xi]);
-        obj += tab[t * stride + c];
-      }
-  }
-}
-
-template <class PR>
-__device__ __forceinline__ void shell_terms_table(const PR &pr, uint32_t s, const uint32_t *cc,
-                                                  double &obj) {
-  switch (pr.K) {
-    case 2: return shell_terms_table_k<2>(pr, s, cc, obj, 2);
-    case 3: return shell_terms_table_k<3>(pr, s, cc, obj, 3);
-    case 4: return shell_terms_table_k<4>(pr, s, cc, obj, 4);
-    case 5: return shell_terms_table_k<5>(pr, s, cc, obj, 5);
-    case 6: return shell_terms_table_k<6>(pr, s, cc, obj, 6);
-    default: return shell_terms_table_k<0>(pr, s, cc, obj, pr.K);
-  }
-}
-
 // Search kernels with one one-hot column fewer than species (DevProblem::Kc = K - 1): the
 // last species L = K - 1 has no column, and cc holds the (K-1) x (K-1) counts C[a][b] of the
 // others.  With F[a][b] = C[a][b] + C[b][a] (F[a][a] = 2 C[a][a]) the bonds at the sites of
@@ -309,9 +257,8 @@ __device__ __forceinline__ void shell_terms_table(const PR &pr, uint32_t s, cons
 //     c(a, L) = F[a][L] = D[a] - sum_{b < L} F[a][b]
 //     c(L, L) = (D[L] - sum_{a < L} F[a][L]) / 2
 // exactly, and the terms follow from the same counts as with all K columns, in the same
-// order (optimization.h:85-100).  TABLE: term values from the job's tables (a count is
-// clamped to the table's range, which only a wrong D could leave); else the arithmetic.
-template <int KT, bool TABLE, class PR>
+// order (optimization.h:85-100).
+template <int KT, class PR>
 __device__ __forceinline__ void shell_terms_reduced_k(const PR &pr, uint32_t s,
                                                       const uint32_t *cc, double &obj) {
   constexpr int L = KT - 1, NT = KT * (KT + 1) / 2;
@@ -334,38 +281,24 @@ __device__ __forceinline__ void shell_terms_reduced_k(const PR &pr, uint32_t s,
         c[t] = (constant_ref(pr.dsum, s * KT + L) - fl) >> 1;
       }
     }
-  if constexpr (TABLE) {
-    const uint32_t at = constant_ref(pr.otab_shell, 2 * s);
-    const uint32_t stride = constant_ref(pr.otab_shell, 2 * s + 1);
-    const double *tab = pr.otab + at;
-    double v[NT];
+  const TermParam *tp = pr.tparams + s * NT;
 #pragma unroll
-    for (int u = 0; u < NT; ++u) {
-      const uint32_t cu = c[u] < stride ? c[u] : stride - 1;
-      v[u] = tab[static_cast<uint32_t>(u) * stride + cu];
-    }
-#pragma unroll
-    for (int u = 0; u < NT; ++u) obj += v[u];
-  } else {
-    const TermParam *tp = pr.tparams + s * NT;
-#pragma unroll
-    for (int u = 0; u < NT; ++u) {
-      const auto &e = constant_ref(tp, u);
-      const double v = 1.0 - static_cast<double>(c[u]) * e.pref;
-      obj += e.pw * absolute(v - e.tgt);
-    }
+  for (int u = 0; u < NT; ++u) {
+    const auto &e = constant_ref(tp, u);
+    const double v = 1.0 - static_cast<double>(c[u]) * e.pref;
+    obj += e.pw * absolute(v - e.tgt);
   }
 }
 
-template <bool TABLE, class PR>
+template <class PR>
 __device__ __forceinline__ void shell_terms_reduced(const PR &pr, uint32_t s,
                                                     const uint32_t *cc, double &obj) {
   switch (pr.K) {  // the host reduces problems of 2-6 species only
-    case 2: return shell_terms_reduced_k<2, TABLE>(pr, s, cc, obj);
-    case 3: return shell_terms_reduced_k<3, TABLE>(pr, s, cc, obj);
-    case 4: return shell_terms_reduced_k<4, TABLE>(pr, s, cc, obj);
-    case 5: return shell_terms_reduced_k<5, TABLE>(pr, s, cc, obj);
-    default: return shell_terms_reduced_k<6, TABLE>(pr, s, cc, obj);
+    case 2: return shell_terms_reduced_k<2>(pr, s, cc, obj);
+    case 3: return shell_terms_reduced_k<3>(pr, s, cc, obj);
+    case 4: return shell_terms_reduced_k<4>(pr, s, cc, obj);
+    case 5: return shell_terms_reduced_k<5>(pr, s, cc, obj);
+    default: return shell_terms_reduced_k<6>(pr, s, cc, obj);
   }
 }
 
@@ -378,9 +311,8 @@ __device__ __forceinline__ void shell_terms_reduced(const PR &pr, uint32_t s,
 // of it (bit set in `active`) has a partial objective key at or below prune_key: every term
 // is >= 0 (the host checks the pair weights), so partial sums only grow and such candidates
 // can never be kept.  Returns the objective of the lane's own candidate (lane < npass * CW)
-// and, when given, writes its ordered bonds and SRO cube; TABLE: the objective from the
-// job's term tables (no bonds or SRO output).
-template <int TT, int G, int D, bool TABLE, class PR>
+// and, when given, writes its ordered bonds and SRO cube.
+template <int TT, int G, int D, class PR>
 __device__ __forceinline__ double dense_score(const PR &pr, uint32_t Kc, const uint8_t *rows,
                                               uint32_t stride, uint32_t *cnt,
                                               uint32_t CW, uint32_t npass,
@@ -418,9 +350,7 @@ __device__ __forceinline__ double dense_score(const PR &pr, uint32_t Kc, const u
   auto terms_of = [&](uint32_t opass, uint32_t os) {
     if (my_pass == opass) {
       if (reduced) {
-        shell_terms_reduced<TABLE>(pr, os, cnt + my_cc * cs, obj);
-      } else if constexpr (TABLE) {
-        shell_terms_table(pr, os, cnt + my_cc * cs, obj);
+        shell_terms_reduced(pr, os, cnt + my_cc * cs, obj);
       } else {
         const uint32_t cr = static_cast<uint32_t>(lane);  // candidate row
         const bool out = cr < out_rows;
@@ -550,9 +480,8 @@ __device__ __forceinline__ void dense_segment(const RP &p, uint64_t sbeg, uint64
       prune_key = g0 < l0 ? g0 : l0;
     }
     const double tot =
-        dense_score<TT, G, D, kDenseTermTables<TT>>(pr, pr.Kc, rows, stride, cnt, CW, npass,
-                                                    live, lane, nullptr, nullptr, 0u, am,
-                                                    prune_key);
+        dense_score<TT, G, D>(pr, pr.Kc, rows, stride, cnt, CW, npass, live, lane, nullptr,
+                              nullptr, 0u, am, prune_key);
     scored += static_cast<unsigned long long>(__popcll(am));
     // a candidate above the threshold, or above the filter's L-th level when its turn
     // comes, cannot be inserted
@@ -607,7 +536,7 @@ __device__ __forceinline__ void dense_segment_split(const RP &p, uint64_t sbeg, 
   wave_sync();
   double tot = 0.0;
   for (uint32_t q = 0; q < nprob; ++q) {
-    const double o = dense_score<TT, G, D, kDenseTermTables<TT>>(
+    const double o = dense_score<TT, G, D>(
         p.prob[q], p.prob[q].Kc, wl + p.off_rows[q], stride, cnt, CW, npass, live, lane, nullptr,
         nullptr, 0u, am, ~0ull);
     subs[q * 64 + static_cast<uint32_t>(lane)] = o;
@@ -761,7 +690,7 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
   wave_sync();
   const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * GEN;
   const uint32_t out_rows = static_cast<uint32_t>(p.B - b0 < GEN ? p.B - b0 : GEN);
-  const double obj = dense_score<TT, G, D, false>(
+  const double obj = dense_score<TT, G, D>(
       pr, pr.K, rows, p.dense_stride, cnt, CW, npass, live, lane,
       p.bonds ? p.bonds + b0 * nb : nullptr, p.sro ? p.sro + b0 * nb : nullptr, out_rows);
   if (active) p.objective[b] = obj;

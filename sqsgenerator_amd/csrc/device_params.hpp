@@ -35,10 +35,6 @@ struct DevProblem {
   // row tile, column tile) order
   const uint32_t *dtiles;     // 256 words per tile
   const uint8_t *species0D;   // sorted configuration, 32 * dT bytes, padding sites 0xff
-  // objective term values by bond count (dense search kernels): shell s starts at
-  // otab[otab_shell[2s]], term t of it at + t * otab_shell[2s + 1] (P_s + 1 counts)
-  const double *otab;
-  const uint32_t *otab_shell;
   // one-hot columns per candidate of the dense search kernels: K, or K - 1 when the bonds of
   // the last species follow from the others' (every shuffle window's sites have equal
   // neighbour counts in each shell); then dsum[s][a] is the sum of the shell-s neighbour

@@ -300,31 +300,6 @@ void Problem::build_dense(uint32_t TT) {
   std::copy(species_packed.begin(), species_packed.end(), sp.begin());
   upload(d_dtiles, tiles, 16);
   upload(d_species0D, sp, 16);
-  // term values by bond count, with the operations of the kernels' objective in the
-  // same order (optimization.h:85-100): v = 1 - c * pref, then pw * absolute(v - tgt);
-  // the search kernels of tile bounds up to 8 read them
-  if (TT > 8) return;
-  std::vector<uint32_t> per_shell(S, 0);
-  for (uint32_t s : ps) ++per_shell[s];
-  std::vector<double> tab;
-  std::vector<uint32_t> shell_at(2 * size_t(S));
-  for (uint32_t s = 0; s < S; ++s) {
-    const uint32_t stride = per_shell[s] + 1;
-    shell_at[2 * s] = static_cast<uint32_t>(tab.size());
-    shell_at[2 * s + 1] = stride;
-    for (uint32_t xi = 0; xi < K; ++xi)
-      for (uint32_t eta = xi; eta < K; ++eta) {
-        const size_t ix = (size_t(s) * K + xi) * K + eta;
-        for (uint32_t c = 0; c < stride; ++c) {
-          const double v = 1.0 - static_cast<double>(c) * pref[ix];
-          const double a = v - tgt[ix];
-          tab.push_back(pw[ix] * (a > 0 ? a : -a));  // absolute (numeric.h:15)
-        }
-      }
-  }
-  otab_n = tab.size();
-  upload(d_otab, tab);
-  upload(d_otab_shell, shell_at);
 }
 
 void ensure_dense(sqsg_job *job) {
@@ -408,8 +383,6 @@ DevProblem Problem::dev() const {
   d.F = F;
   d.dtiles = d_dtiles.as<const uint32_t>();
   d.species0D = d_species0D.as<const uint8_t>();
-  d.otab = d_otab.as<const double>();
-  d.otab_shell = d_otab_shell.as<const uint32_t>();
   d.dsum = d_dsum.as<const uint32_t>();
   d.Kc = Kc ? Kc : K;
   d.dT = dT;

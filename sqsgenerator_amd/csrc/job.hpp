@@ -89,11 +89,6 @@ struct Problem {
   uint32_t dtiles_n = 0;
   uint32_t dtiles_nonzero = 0;  // tiles holding at least one pair
   DevBuf d_dtiles, d_species0D;
-  // objective term tables of the dense search kernels: per shell s with P_s pairs and each
-  // of its terms t (reference order), the P_s + 1 values pw * absolute(1 - c * pref - tgt)
-  // of the bond count c = c_ab + c_ba; per shell its first value and its term stride
-  size_t otab_n = 0;
-  DevBuf d_otab, d_otab_shell;
   // one-hot columns of the dense search kernels (K, or K - 1: DevProblem::Kc) and the
   // per-shell, per-species neighbour-count sums that let the last species go without one
   uint32_t Kc = 0;
