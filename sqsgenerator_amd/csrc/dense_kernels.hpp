@@ -227,7 +227,7 @@ __device__ __forceinline__ void shell_terms_k(const PR &pr, uint32_t s, const ui
       const uint32_t cab = cc[xi * K + eta];
       const uint32_t cba = xi == eta ? 0u : cc[eta * K + xi];
       const double v = 1.0 - static_cast<double>(cab + cba) * e.pref;
-      obj += e.pw * absolute(v - e.tgt);
+      obj += e.pw * term_abs(v - e.tgt);
       if (sro) sro[(s * K + xi) * K + eta] = v, sro[(s * K + eta) * K + xi] = v;
       if (bonds) {
         bonds[(s * K + xi) * K + eta] = static_cast<int32_t>(cab);
@@ -286,7 +286,7 @@ __device__ __forceinline__ void shell_terms_reduced_k(const PR &pr, uint32_t s,
   for (int u = 0; u < NT; ++u) {
     const auto &e = constant_ref(tp, u);
     const double v = 1.0 - static_cast<double>(c[u]) * e.pref;
-    obj += e.pw * absolute(v - e.tgt);
+    obj += e.pw * term_abs(v - e.tgt);
   }
 }
 

@@ -27,6 +27,13 @@ __device__ __forceinline__ uint32_t draw32(uint64_t &st) {
 
 __device__ __forceinline__ double absolute(double a) { return a > 0 ? a : -a; }  // numeric.h:15
 
+// |v - tgt| of an objective term as an accumulated objective needs it.  absolute() returns
+// -0.0 for +0.0 where fabs returns +0.0, and nothing else differs for finite values.  A
+// term's zero sign never changes the sum it joins: the sum starts at +0.0, so it is never
+// -0.0 (an exact zero sum rounds to +0.0), and x + 0 = x for either zero.  fabs folds into
+// the following multiply as a source modifier instead of a compare and select.
+__device__ __forceinline__ double term_abs(double a) { return __builtin_fabs(a); }
+
 __device__ __forceinline__ unsigned long long okey(double d) {
   const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(d));
   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);

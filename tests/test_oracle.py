@@ -302,6 +302,36 @@ def test_objective_summation_order():
     assert o == 0.0
 
 
+def test_term_zero_sign_never_changes_the_objective():
+    """the dense kernels take |v - tgt| as fabs where the reference's absolute (numeric.h:15)
+    gives -0.0 for +0.0: sums from +0.0 of the same terms with either zero sign are equal bit
+    for bit, including all-zero sums, exact cancellations and negative pair weights"""
+    rng = np.random.default_rng(5)
+
+    def ref_abs(a):
+        return a if a > 0 else -a
+
+    def total(terms_v, pw, absf):
+        obj = 0.0
+        for v, w in zip(terms_v, pw):
+            obj += w * absf(v)
+        return obj
+
+    cases = [([0.0, 0.0, 0.0], [1.0, -2.0, 0.5]),  # every term a zero
+             ([0.25, 0.0, -0.25], [4.0, 3.0, -4.0]),  # cancels exactly to zero, then a zero
+             ([0.0, 0.125, 0.0], [-1.0, -8.0, 2.0])]
+    for _ in range(2000):
+        n = int(rng.integers(1, 30))
+        v = rng.choice([0.0, -0.0, 0.5, -0.5, 1.0 / 3.0], size=n) * rng.choice([1.0, 1e-300, 3.0],
+                                                                             size=n)
+        w = rng.choice([1.0, -1.0, 0.5, 2.0, 0.0], size=n)
+        cases.append((list(v), list(w)))
+    for v, w in cases:
+        a = np.float64(total(v, w, ref_abs)).view(np.uint64)
+        b = np.float64(total(v, w, math.fabs)).view(np.uint64)
+        assert a == b, (v, w)
+
+
 # ---- driver: result collection semantics (sqs.h:348-455, results.h:42-99) --------------
 
 def _reference_test_16_problem(pair_weights=None):
