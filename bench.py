@@ -311,8 +311,13 @@ def batched_uneven(iterations: int) -> dict:
                             "alone_candidates_per_s": rep["candidates"] / wall,
                             "alone_kernel_ms": rep["kernel_ms"], "alone_steps": rep["launches"]})
         job.close()
-    # the batched run of all four, three times over fresh jobs (a run lasts ~15 ms, so the
+    # the batched run of all four, once untimed (the engine's groups, graphs and pooled
+    # buffers for this mix), then three times over fresh jobs (a run lasts ~15 ms, so the
     # median of three is reported)
+    warm = [make(fn) for _, fn in makers]
+    timed(warm)
+    for j in warm:
+        j.close()
     runs = []
     for _ in range(3):
         jobs = [make(fn) for _, fn in makers]
