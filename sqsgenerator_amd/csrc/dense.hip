@@ -64,7 +64,8 @@ uint32_t dense_shell_tile_lds(uint32_t tile_bound) {
   uint32_t b = 0;
   (void)with_dense_shape(tile_bound, [&](auto sh) {
     using Sh = decltype(sh);
-    b = (shell_lds_tiles<Sh>() ? Sh::TT * (Sh::TT + 1) / 2 : kShellBufs * kShellChunk) * 1024u;
+    b = (shell_lds_tiles<Sh>() ? Sh::TT * (Sh::TT + 1) / 2
+                                : kShellBufs * shell_chunk<Sh::TT, Sh::G>()) * 1024u;
     return hipSuccess;
   });
   return b;

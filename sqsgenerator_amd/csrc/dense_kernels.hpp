@@ -702,7 +702,7 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
 // wavefronts scores one shell s (blockIdx.y) for kShellWaves passes of CW candidates (one
 // pass per wavefront).  The block's wavefronts share the shell's tiles in LDS: LT (tile
 // bounds up to 8, a shell's tiles at most 36 KiB) copies them once; larger bounds stream them
-// through kShellBufs LDS buffers of kShellChunk grid positions: the loads of chunk k + 3
+// through kShellBufs LDS buffers of shell_chunk() grid positions: the loads of chunk k + 3
 // are issued as chunk k starts (two chunks in registers at a time) and chunk k + 2 is stored
 // as chunk k ends, then one barrier.  Read from LDS, a tile costs 1 KiB of LDS bandwidth per
 // MFMA group; read by each wavefront from the L2 stream it would cost the same in L1
@@ -718,15 +718,20 @@ __global__ __launch_bounds__(64) void score_dense_kernel(ScoreParams p) {
 // slower; 16-bit bond counts instead of the values move the term arithmetic into the sum
 // kernel's serial per-candidate loop, 1.4x slower.
 constexpr uint32_t kShellWaves = 4;
-constexpr int kShellChunk = 16, kShellBufs = 3;
+constexpr int kShellBufs = 3;
+// streamed form: one tile group per wavefront fits three blocks per CU in registers and, with
+// chunks of 8 positions, in LDS (the 1197 blocks of 864 sites x 4096 candidates then run in
+// two rounds instead of three); two groups keep chunks of 16 and one block per SIMD pair
+template <int TT, int G> constexpr bool shell_three_blocks() { return G == 1 && TT <= 27; }
+template <int TT, int G> constexpr int shell_chunk() { return shell_three_blocks<TT, G>() ? 8 : 16; }
 
 template <int TT, int G, int D, bool LT>
-__global__ __launch_bounds__(64 * kShellWaves, LT ? 2 : 1) void score_shell_kernel(ScoreParams p,
-                                                                                   double *vals) {
+__global__ __launch_bounds__(64 * kShellWaves, LT ? 2 : (shell_three_blocks<TT, G>() ? 3 : 1))
+void score_shell_kernel(ScoreParams p, double *vals) {
   extern __shared__ __align__(16) uint8_t lds[];
   constexpr int W = static_cast<int>(kShellWaves);
   constexpr int P = TT * (TT + 1) / 2;
-  constexpr int Q = kShellChunk, NCK = (P + Q - 1) / Q;  // streamed form: chunks of Q tiles
+  constexpr int Q = shell_chunk<TT, G>(), NCK = (P + Q - 1) / Q;  // streamed: chunks of Q tiles
   constexpr int PER = Q * 64 / (64 * W);  // 16-byte loads of a chunk per thread
   static_assert(LT || (Q * 64) % (64 * W) == 0, "a chunk splits evenly over the block");
   constexpr uint32_t kTileLds = static_cast<uint32_t>(LT ? P : kShellBufs * Q) * 1024u;
