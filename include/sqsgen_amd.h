@@ -51,6 +51,9 @@ extern "C" {
 #define SQSG_ITERATION_SYSTEMATIC 1
 #define SQSG_SUBLATTICE_INTERACT 0
 #define SQSG_SUBLATTICE_SPLIT 1
+/* precision of the configuration (io/config/combined.h:151-159, configuration<float|double>) */
+#define SQSG_PREC_DOUBLE 0
+#define SQSG_PREC_SINGLE 1
 
 /* scoring kernel selectors for sqsg_score_batch / sqsg_job_set_kernel.  All variants give
  * identical results (bit-exact bonds, objectives and result packs).  AUTO runs the dense
@@ -94,13 +97,28 @@ SQSG_API int sqsg_prefactors(const uint32_t *shell_matrix, uint32_t n, const uin
 SQSG_API int sqsg_supercell(const double *lattice, const double *frac, const uint8_t *species,
                             uint32_t n, uint32_t a, uint32_t b, uint32_t c, double *lattice_out,
                             double *frac_out, uint8_t *species_out);
+/* the same geometry in float, for configuration<float> (structure<float>) */
+SQSG_API int sqsg_distance_matrix_f32(const float *lattice, const float *frac, uint32_t n,
+                                      float *out);
+SQSG_API int sqsg_distances_naive_f32(const float *dm, uint32_t n, float atol, float rtol,
+                                      float *out, uint32_t cap, uint32_t *count);
+SQSG_API int sqsg_distances_histogram_f32(const float *dm, uint32_t n, float bin_width,
+                                          float peak_isolation, float *out, uint32_t cap,
+                                          uint32_t *count);
+SQSG_API int sqsg_shell_matrix_f32(const float *dm, uint32_t n, const float *radii,
+                                   uint32_t num_radii, float atol, float rtol, uint32_t *out);
+SQSG_API int sqsg_prefactors_f32(const uint32_t *shell_matrix, uint32_t n, const uint32_t *shells,
+                                 uint32_t num_shells, const uint8_t *species_z, float *out);
+SQSG_API int sqsg_supercell_f32(const float *lattice, const float *frac, const uint8_t *species,
+                                uint32_t n, uint32_t a, uint32_t b, uint32_t c, float *lattice_out,
+                                float *frac_out, uint8_t *species_out);
 
 /* ---- multiset permutations (include/sqsgen/core/permutation.h) ------------------------ */
 SQSG_API int sqsg_num_permutations(const uint8_t *conf, uint32_t n, char *out, size_t cap);
 SQSG_API int sqsg_rank_permutation(const uint8_t *conf, uint32_t n, char *out, size_t cap);
 SQSG_API int sqsg_unrank_permutation(uint8_t *conf, uint32_t n, const char *rank);
 
-/* ---- job description: core/config.h:32-48 configuration<T> (T = double) ------------- */
+/* ---- job description: core/config.h:32-48 configuration<T> (T = double or float) ---- */
 typedef struct sqsg_config {
   /* structure after the supercell was applied (config.h:15-30) */
   uint32_t num_sites;
@@ -133,6 +151,11 @@ typedef struct sqsg_config {
   const uint8_t *seed_present;     /* per problem; absent -> random_device seed */
   uint64_t keep;
   int64_t max_results_per_objective; /* <0: unlimited */
+  /* SQSG_PREC_SINGLE: configuration<float>.  Every floating-point input above must then hold
+   * float values (stored as double); geometry, scaled pair weights and objectives are
+   * evaluated in float (sqs.h:591-634), results hold float values as double.  Such jobs
+   * run on the sparse kernels (wave, lane). */
+  int32_t precision;
 } sqsg_config;
 
 /* optimization_config<T,Mode>::from_config (optimization_config.h:59-141) plus the

@@ -60,6 +60,19 @@ def _load() -> C.CDLL:
     lib.or_count_bonds.argtypes = [u64p, C.c_uint32, u8p, C.c_uint32, C.c_uint32, u64p]
     lib.or_compute_objective.restype = C.c_double
     lib.or_compute_objective.argtypes = [u64p, f64p, f64p, f64p, C.c_uint32, C.c_uint32, f64p]
+    f32p = C.POINTER(C.c_float)
+    lib.or_distance_matrix_f32.argtypes = [f32p, f32p, C.c_uint32, f32p]
+    lib.or_distance_matrix_f32.restype = None
+    lib.or_shell_matrix_f32.argtypes = [f32p, C.c_uint32, f32p, C.c_uint32, C.c_float,
+                                        C.c_float, u32p]
+    lib.or_shell_matrix_f32.restype = None
+    lib.or_compute_prefactors_f32.argtypes = [u32p, C.c_uint32, u32p, C.c_uint32, u8p, f32p]
+    lib.or_compute_prefactors_f32.restype = C.c_uint32
+    lib.or_scaled_pair_weights_f32.argtypes = [f32p, f32p, C.c_uint32, C.c_uint32]
+    lib.or_scaled_pair_weights_f32.restype = None
+    lib.or_compute_objective_f32.argtypes = [u64p, f32p, f32p, f32p, C.c_uint32, C.c_uint32,
+                                             f32p]
+    lib.or_compute_objective_f32.restype = C.c_float
     lib.or_run.argtypes = [C.POINTER(RunConfig), C.POINTER(RunStats)]
     lib.or_result_count.restype = C.c_uint64
     lib.or_result_get.argtypes = [C.c_uint64, f64p, u8p, f64p]
@@ -339,3 +352,62 @@ def run(problems, iterations, chunk_size=None, systematic=False, split=False, ke
             lib().or_result_get(r, C.byref(o), _p(sp, u8p), _p(so, f64p))
             results.append((o.value, sp.copy(), so.copy()))
     return st, results
+
+
+# ---- configuration<float>: the same functions with T = float ---------------------------
+
+_f32p = C.POINTER(C.c_float)
+
+
+def distance_matrix_f32(lattice, frac) -> np.ndarray:
+    """structure.h:34-69, T = float"""
+    L = np.ascontiguousarray(lattice, dtype=np.float32).reshape(3, 3)
+    f = np.ascontiguousarray(frac, dtype=np.float32).reshape(-1, 3)
+    n = len(f)
+    out = np.empty((n, n), dtype=np.float32)
+    lib().or_distance_matrix_f32(_p(L, _f32p), _p(f, _f32p), n, _p(out, _f32p))
+    return out
+
+
+def shell_matrix_f32(dm, radii, atol=float(np.finfo(np.float32).eps), rtol=1e-9) -> np.ndarray:
+    """structure.h:71-100, T = float"""
+    d = np.ascontiguousarray(dm, dtype=np.float32)
+    r = np.ascontiguousarray(radii, dtype=np.float32)
+    n = d.shape[0]
+    out = np.empty((n, n), dtype=np.uint32)
+    lib().or_shell_matrix_f32(_p(d, _f32p), n, _p(r, _f32p), len(r), atol, rtol, _p(out, u32p))
+    return out
+
+
+def compute_prefactors_f32(sm, shells, species_z) -> np.ndarray:
+    """structure.h:136-192, T = float"""
+    s = np.ascontiguousarray(sm, dtype=np.uint32)
+    sh = np.ascontiguousarray(sorted(shells), dtype=np.uint32)
+    z = np.ascontiguousarray(species_z, dtype=np.uint8)
+    K = len(set(z.tolist()))
+    out = np.zeros((len(sh), K, K), dtype=np.float32)
+    lib().or_compute_prefactors_f32(_p(s, u32p), s.shape[0], _p(sh, u32p), len(sh), _p(z, u8p),
+                                    _p(out, _f32p))
+    return out
+
+
+def scaled_pair_weights_f32(pw, shell_weights_sorted) -> np.ndarray:
+    """optimization.h:17-33, T = float"""
+    w = np.ascontiguousarray(pw, dtype=np.float32).copy()
+    sw = np.ascontiguousarray(shell_weights_sorted, dtype=np.float32)
+    S, K, _ = w.shape
+    lib().or_scaled_pair_weights_f32(_p(w, _f32p), _p(sw, _f32p), S, K)
+    return w
+
+
+def compute_objective_f32(bonds, prefactors, pair_weights, target):
+    """optimization.h:79-103, T = float -> (objective, sro cube) as float32"""
+    b = np.ascontiguousarray(bonds, dtype=np.uint64)
+    S, K, _ = b.shape
+    pf = np.ascontiguousarray(prefactors, dtype=np.float32)
+    pw = np.ascontiguousarray(pair_weights, dtype=np.float32)
+    tg = np.ascontiguousarray(target, dtype=np.float32)
+    sro = np.zeros((S, K, K), dtype=np.float32)
+    o = lib().or_compute_objective_f32(_p(b, u64p), _p(pf, _f32p), _p(pw, _f32p), _p(tg, _f32p),
+                                       S, K, _p(sro, _f32p))
+    return np.float32(o), sro

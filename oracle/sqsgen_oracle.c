@@ -435,6 +435,141 @@ double or_compute_objective(const uint64_t *bonds, const double *pref, const dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* configuration<float> (sqs.h:591-597): the same restatements with T = float.
+ * C evaluates float operands in float (FLT_EVAL_METHOD 0 on x86-64 SSE), as the
+ * reference's float instantiation does; literals follow the reference's types. */
+
+static int is_close_f(float a, float b, float atol, float rtol) { /* numeric.h:11-13 */
+  return fabsf(a - b) <= (atol + rtol * fabsf(b));
+}
+
+/* structure.h:34-69 with T = float */
+void or_distance_matrix_f32(const float *L, const float *frac, uint32_t n, float *out) {
+  float *cart = (float *)malloc(sizeof(float) * 3 * (n ? n : 1));
+  for (uint32_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k)
+      cart[3 * i + k] = frac[3 * i + 0] * L[0 * 3 + k] + frac[3 * i + 1] * L[1 * 3 + k]
+                        + frac[3 * i + 2] * L[2 * 3 + k];
+  for (uint64_t q = 0; q < (uint64_t)n * n; ++q) out[q] = FLT_MAX;
+  const int axis[3] = {-1, 0, 1};
+  for (uint32_t i = 0; i < n; ++i) {
+    const float *p1 = cart + 3 * i;
+    for (uint32_t j = i; j < n; ++j) {
+      const float *p2 = cart + 3 * j;
+      for (int iu = 0; iu < 3; ++iu)
+        for (int iv = 0; iv < 3; ++iv)
+          for (int iw = 0; iw < 3; ++iw) {
+            float u = (float)axis[iu], v = (float)axis[iv], w = (float)axis[iw];
+            float d[3];
+            for (int k = 0; k < 3; ++k) {
+              float t = u * L[0 * 3 + k] + v * L[1 * 3 + k] + w * L[2 * 3 + k];
+              d[k] = p1[k] - (t + p2[k]);
+            }
+            float norm = fabsf(sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+            if (norm < out[(uint64_t)i * n + j]) {
+              out[(uint64_t)i * n + j] = norm;
+              if (i != j) out[(uint64_t)j * n + i] = norm;
+            }
+          }
+    }
+  }
+  free(cart);
+}
+
+/* structure.h:71-100 with T = float */
+void or_shell_matrix_f32(const float *dm, uint32_t n, const float *radii, uint32_t num_radii,
+                         float atol, float rtol, uint32_t *out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    out[(uint64_t)i * n + i] = 0;
+    for (uint32_t j = i + 1; j < n; ++j) {
+      float d = dm[(uint64_t)i * n + j];
+      uint32_t shell = num_radii;
+      if (is_close_f(d, 0.0f, atol, rtol))
+        shell = 0;
+      else {
+        for (uint32_t r = 0; r + 1 < num_radii; ++r) {
+          float lb = radii[r], up = radii[r + 1];
+          if ((is_close_f(d, lb, atol, rtol) || d > lb) &&
+              (is_close_f(d, up, atol, rtol) || up > d)) {
+            shell = r + 1;
+            break;
+          }
+        }
+      }
+      out[(uint64_t)i * n + j] = shell;
+      out[(uint64_t)j * n + i] = shell;
+    }
+  }
+}
+
+/* structure.h:136-192 with T = float */
+uint32_t or_compute_prefactors_f32(const uint32_t *sm, uint32_t n, const uint32_t *shells,
+                                   uint32_t num_shells, const uint8_t *species_z, float *out) {
+  uint64_t zcount[256] = {0};
+  for (uint32_t i = 0; i < n; ++i) zcount[species_z[i]]++;
+  uint8_t zlist[256];
+  uint32_t K = 0;
+  for (int z = 0; z < 256; ++z)
+    if (zcount[z]) zlist[K++] = (uint8_t)z;
+  for (uint32_t s = 0; s < num_shells; ++s) {
+    uint64_t cnt = 0;
+    for (uint64_t q = 0; q < (uint64_t)n * n; ++q) cnt += (sm[q] == shells[s]);
+    uint64_t neighbor_count = 0;
+    if (cnt > 0) neighbor_count = (uint64_t)((float)cnt / (float)n);
+    for (uint32_t a = 0; a < K; ++a)
+      for (uint32_t b = a; b < K; ++b) {
+        float pref = 0.0f;
+        if (neighbor_count > 0) {
+          float M = (float)neighbor_count;
+          float xa = (float)zcount[zlist[a]] / (float)n;
+          float xb = (float)zcount[zlist[b]] / (float)n;
+          pref = 1.0f / (M * xa * xb * (float)n);
+        }
+        out[(s * K + a) * K + b] = pref;
+        out[(s * K + b) * K + a] = pref;
+      }
+  }
+  return K;
+}
+
+/* optimization.h:17-33 with T = float */
+void or_scaled_pair_weights_f32(float *pw, const float *w, uint32_t S, uint32_t K) {
+  for (uint32_t s = 0; s < S; ++s)
+    for (uint32_t xi = 0; xi < K; ++xi)
+      for (uint32_t eta = xi; eta < K; ++eta) {
+        pw[(s * K + xi) * K + eta] *= w[s];
+        if (xi == eta) continue;
+        pw[(s * K + eta) * K + xi] *= w[s];
+      }
+}
+
+/* optimization.h:79-103 with T = float (absolute() of numeric.h:15 in float) */
+float or_compute_objective_f32(const uint64_t *bonds, const float *pref, const float *pw,
+                               const float *tgt, uint32_t S, uint32_t K, float *sro) {
+  float objective = 0.0f;
+#define IX(s, a, b) (((s) * K + (a)) * K + (b))
+  for (uint32_t s = 0; s < S; ++s)
+    for (uint32_t xi = 0; xi < K; ++xi) {
+      float sigma = 1.0f - (float)bonds[IX(s, xi, xi)] * pref[IX(s, xi, xi)];
+      if (sro) sro[IX(s, xi, xi)] = sigma;
+      float a = sigma - tgt[IX(s, xi, xi)];
+      objective += pw[IX(s, xi, xi)] * (a > 0 ? a : -a);
+      for (uint32_t eta = xi + 1; eta < K; ++eta) {
+        uint64_t pb = bonds[IX(s, xi, eta)] + bonds[IX(s, eta, xi)];
+        float v = 1.0f - (float)pb * pref[IX(s, xi, eta)];
+        if (sro) {
+          sro[IX(s, xi, eta)] = v;
+          sro[IX(s, eta, xi)] = v;
+        }
+        float b = v - tgt[IX(s, xi, eta)];
+        objective += pw[IX(s, xi, eta)] * (b > 0 ? b : -b);
+      }
+    }
+#undef IX
+  return objective;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Driver restatement: sqs.h:299-467 worker loop, results.h:42-99 collection. */
 
 typedef struct {

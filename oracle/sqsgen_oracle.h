@@ -32,6 +32,10 @@
  *   or_compute_prefactors, or_scaled_pair_weights  closed forms
  *   or_count_bonds         brute-force dense contraction X^T U X; per-shell totals = pairs
  *   or_compute_objective   closed forms, zero weights, unit prefactors, summation order
+ *   or_*_f32               configuration<float>: the same statements with T = float; pinned
+ *                          by the same closed forms and invariants in float, by agreement
+ *                          with the double functions to float rounding, and by the FCC
+ *                          coordination numbers from the float shell matrix
  *   or_run                 systematic: all 12870 results with zero pair weights; keep
  *                          levels hold every tie; max_results_per_objective allows max+1;
  *                          the whole collection (every level, never evicted, the stall of
@@ -91,6 +95,18 @@ void or_count_bonds(const uint64_t *pairs, uint32_t num_pairs, const uint8_t *sp
 double or_compute_objective(const uint64_t *bonds, const double *prefactors,
                             const double *pair_weights, const double *target, uint32_t num_shells,
                             uint32_t num_species, double *sro);
+
+/* ---- configuration<float> (T = float) restatements of the same functions ---- */
+void or_distance_matrix_f32(const float *lattice, const float *frac, uint32_t n, float *out);
+void or_shell_matrix_f32(const float *dm, uint32_t n, const float *radii, uint32_t num_radii,
+                         float atol, float rtol, uint32_t *out);
+uint32_t or_compute_prefactors_f32(const uint32_t *sm, uint32_t n, const uint32_t *shells,
+                                   uint32_t num_shells, const uint8_t *species_z, float *out);
+void or_scaled_pair_weights_f32(float *pw, const float *shell_weights_sorted,
+                                uint32_t num_shells, uint32_t num_species);
+float or_compute_objective_f32(const uint64_t *bonds, const float *prefactors,
+                               const float *pair_weights, const float *target,
+                               uint32_t num_shells, uint32_t num_species, float *sro);
 
 /* ---- driver: sqs.h:226-238,299-467, core/results.h:42-99 ---- */
 typedef struct {

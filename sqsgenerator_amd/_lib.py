@@ -26,6 +26,7 @@ u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
 i32p = C.POINTER(C.c_int32)
 f64p = C.POINTER(C.c_double)
+f32p = C.POINTER(C.c_float)
 
 
 class SqsgError(RuntimeError):
@@ -53,6 +54,7 @@ class Config(C.Structure):
         ("pair_weights", f64p), ("target_objective", f64p), ("iterations", C.c_uint64),
         ("chunk_size", C.c_uint64), ("seeds", u64p), ("seed_present", u8p),
         ("keep", C.c_uint64), ("max_results_per_objective", C.c_int64),
+        ("precision", C.c_int32),
     ]
 
 
@@ -98,6 +100,16 @@ SIGNATURES = {
     "sqsg_prefactors": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, u8p, f64p]),
     "sqsg_supercell": (C.c_int, [f64p, f64p, u8p, C.c_uint32, C.c_uint32, C.c_uint32,
                                  C.c_uint32, f64p, f64p, u8p]),
+    "sqsg_distance_matrix_f32": (C.c_int, [f32p, f32p, C.c_uint32, f32p]),
+    "sqsg_distances_naive_f32": (C.c_int, [f32p, C.c_uint32, C.c_float, C.c_float, f32p,
+                                           C.c_uint32, u32p]),
+    "sqsg_distances_histogram_f32": (C.c_int, [f32p, C.c_uint32, C.c_float, C.c_float, f32p,
+                                               C.c_uint32, u32p]),
+    "sqsg_shell_matrix_f32": (C.c_int, [f32p, C.c_uint32, f32p, C.c_uint32, C.c_float,
+                                        C.c_float, u32p]),
+    "sqsg_prefactors_f32": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, u8p, f32p]),
+    "sqsg_supercell_f32": (C.c_int, [f32p, f32p, u8p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_uint32, f32p, f32p, u8p]),
     "sqsg_num_permutations": (C.c_int, [u8p, C.c_uint32, C.c_char_p, C.c_size_t]),
     "sqsg_rank_permutation": (C.c_int, [u8p, C.c_uint32, C.c_char_p, C.c_size_t]),
     "sqsg_unrank_permutation": (C.c_int, [u8p, C.c_uint32, C.c_char_p]),

@@ -30,6 +30,7 @@ CHUNK_SIZE_DEFAULT = 100000    # combined.h:39
 ATOL_DEFAULT, RTOL_DEFAULT = 1.0e-3, 1.0e-5            # shell_radii.h:19-20
 BIN_WIDTH_DEFAULT, PEAK_ISOLATION_DEFAULT = 0.05, 0.25  # shell_radii.h:21-22
 EPS = float(np.finfo(np.float64).eps)
+EPS_F32 = float(np.finfo(np.float32).eps)
 
 random, systematic = "random", "systematic"
 interact, split = "interact", "split"
@@ -68,7 +69,23 @@ def _f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def distance_matrix(lattice, frac_coords) -> np.ndarray:
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _as_prec(x, single: bool):
+    """a value of the configuration's type T (float values are kept as float64)"""
+    return float(np.float32(x)) if single else float(x)
+
+
+def distance_matrix(lattice, frac_coords, single: bool = False) -> np.ndarray:
+    if single:  # structure<float>
+        lat, frac = _f32(lattice).reshape(3, 3), _f32(frac_coords).reshape(-1, 3)
+        n = len(frac)
+        out = np.empty((n, n), dtype=np.float32)
+        L.check(L.lib().sqsg_distance_matrix_f32(L.ptr(lat, L.f32p), L.ptr(frac, L.f32p), n,
+                                                 L.ptr(out, L.f32p)))
+        return out.astype(np.float64)
     lat, frac = _f64(lattice).reshape(3, 3), _f64(frac_coords).reshape(-1, 3)
     n = len(frac)
     out = np.empty((n, n))
@@ -77,7 +94,15 @@ def distance_matrix(lattice, frac_coords) -> np.ndarray:
     return out
 
 
-def distances_naive(dm, atol=EPS, rtol=1e-9) -> np.ndarray:
+def distances_naive(dm, atol=EPS, rtol=1e-9, single: bool = False) -> np.ndarray:
+    if single:
+        d = _f32(dm)
+        n = d.shape[0]
+        out = np.empty(n * n + 1, dtype=np.float32)
+        cnt = C.c_uint32()
+        L.check(L.lib().sqsg_distances_naive_f32(L.ptr(d, L.f32p), n, atol, rtol,
+                                                 L.ptr(out, L.f32p), len(out), C.byref(cnt)))
+        return out[:cnt.value].astype(np.float64)
     d = _f64(dm)
     n = d.shape[0]
     out = np.empty(n * n + 1)
@@ -88,7 +113,16 @@ def distances_naive(dm, atol=EPS, rtol=1e-9) -> np.ndarray:
 
 
 def distances_histogram(dm, bin_width=BIN_WIDTH_DEFAULT,
-                        peak_isolation=PEAK_ISOLATION_DEFAULT) -> np.ndarray:
+                        peak_isolation=PEAK_ISOLATION_DEFAULT, single: bool = False) -> np.ndarray:
+    if single:
+        d = _f32(dm)
+        n = d.shape[0]
+        out = np.empty(n * n + 2, dtype=np.float32)
+        cnt = C.c_uint32()
+        L.check(L.lib().sqsg_distances_histogram_f32(L.ptr(d, L.f32p), n, bin_width,
+                                                     peak_isolation, L.ptr(out, L.f32p),
+                                                     len(out), C.byref(cnt)))
+        return out[:cnt.value].astype(np.float64)
     d = _f64(dm)
     n = d.shape[0]
     out = np.empty(n * n + 2)
@@ -98,7 +132,14 @@ def distances_histogram(dm, bin_width=BIN_WIDTH_DEFAULT,
     return out[:cnt.value].copy()
 
 
-def shell_matrix(dm, radii, atol=EPS, rtol=1e-9) -> np.ndarray:
+def shell_matrix(dm, radii, atol=EPS, rtol=1e-9, single: bool = False) -> np.ndarray:
+    if single:
+        d, r = _f32(dm), _f32(radii)
+        n = d.shape[0]
+        out = np.empty((n, n), dtype=np.uint32)
+        L.check(L.lib().sqsg_shell_matrix_f32(L.ptr(d, L.f32p), n, L.ptr(r, L.f32p), len(r),
+                                              atol, rtol, L.ptr(out, L.u32p)))
+        return out
     d, r = _f64(dm), _f64(radii)
     n = d.shape[0]
     out = np.empty((n, n), dtype=np.uint32)
@@ -107,11 +148,16 @@ def shell_matrix(dm, radii, atol=EPS, rtol=1e-9) -> np.ndarray:
     return out
 
 
-def prefactors(sm, shells, species) -> np.ndarray:
+def prefactors(sm, shells, species, single: bool = False) -> np.ndarray:
     s = np.ascontiguousarray(sm, dtype=np.uint32)
     sh = np.ascontiguousarray(sorted(shells), dtype=np.uint32)
     z = np.ascontiguousarray(species, dtype=np.uint8)
     K = len(set(z.tolist()))
+    if single:
+        outf = np.zeros((len(sh), K, K), dtype=np.float32)
+        L.check(L.lib().sqsg_prefactors_f32(L.ptr(s, L.u32p), s.shape[0], L.ptr(sh, L.u32p),
+                                            len(sh), L.ptr(z, L.u8p), L.ptr(outf, L.f32p)))
+        return outf.astype(np.float64)
     out = np.zeros((len(sh), K, K))
     L.check(L.lib().sqsg_prefactors(L.ptr(s, L.u32p), s.shape[0], L.ptr(sh, L.u32p), len(sh),
                                     L.ptr(z, L.u8p), L.ptr(out, L.f64p)))
@@ -141,9 +187,13 @@ def unrank_permutation(conf, rank: int) -> np.ndarray:
 class Structure:
     """Lattice (rows a, b, c), fractional coordinates and atomic numbers."""
 
-    def __init__(self, lattice, frac_coords, species, pbc=(True, True, True)):
-        self.lattice = _f64(lattice).reshape(3, 3)
-        self.frac_coords = _f64(frac_coords).reshape(-1, 3)
+    def __init__(self, lattice, frac_coords, species, pbc=(True, True, True),
+                 single: bool = False):
+        # structure<float> (single) holds float values, kept as float64
+        self.single = bool(single)
+        cast = (lambda a: _f64(_f32(a))) if self.single else _f64
+        self.lattice = cast(lattice).reshape(3, 3)
+        self.frac_coords = cast(frac_coords).reshape(-1, 3)
         self.species = [to_z(s) for s in species]
         self.pbc = tuple(pbc)
         if len(self.species) != len(self.frac_coords) or not self.species:
@@ -165,6 +215,14 @@ class Structure:
         n = len(self)
         m = n * a * b * c
         z = np.ascontiguousarray(self.species, dtype=np.uint8)
+        if self.single:
+            lat, frac = _f32(self.lattice), _f32(self.frac_coords)
+            lo, fo = np.empty((3, 3), np.float32), np.empty((m, 3), np.float32)
+            zo = np.empty(m, dtype=np.uint8)
+            L.check(L.lib().sqsg_supercell_f32(L.ptr(lat, L.f32p), L.ptr(frac, L.f32p),
+                                               L.ptr(z, L.u8p), n, a, b, c, L.ptr(lo, L.f32p),
+                                               L.ptr(fo, L.f32p), L.ptr(zo, L.u8p)))
+            return Structure(lo, fo, zo.tolist(), self.pbc, True)
         lo, fo, zo = np.empty((3, 3)), np.empty((m, 3)), np.empty(m, dtype=np.uint8)
         L.check(L.lib().sqsg_supercell(L.ptr(self.lattice, L.f64p), L.ptr(self.frac_coords, L.f64p),
                                        L.ptr(z, L.u8p), n, a, b, c, L.ptr(lo, L.f64p),
@@ -172,18 +230,20 @@ class Structure:
         return Structure(lo, fo, zo.tolist(), self.pbc)
 
     def distance_matrix(self) -> np.ndarray:
-        return distance_matrix(self.lattice, self.frac_coords)
+        return distance_matrix(self.lattice, self.frac_coords, self.single)
 
-    def shell_matrix(self, radii, atol=EPS, rtol=1e-9) -> np.ndarray:
-        return shell_matrix(self.distance_matrix(), radii, atol, rtol)
+    def shell_matrix(self, radii, atol=None, rtol=1e-9) -> np.ndarray:
+        if atol is None:  # numeric_limits<T>::epsilon()
+            atol = EPS_F32 if self.single else EPS
+        return shell_matrix(self.distance_matrix(), radii, atol, rtol, self.single)
 
     def sliced(self, indices) -> "Structure":
         idx = list(indices)
         return Structure(self.lattice, self.frac_coords[idx], [self.species[i] for i in idx],
-                         self.pbc)
+                         self.pbc, self.single)
 
     def with_species(self, species) -> "Structure":
-        return Structure(self.lattice, self.frac_coords, species, self.pbc)
+        return Structure(self.lattice, self.frac_coords, species, self.pbc, self.single)
 
     def bytes(self) -> bytes:
         """msgpack wire form (sqsgen.cpp:182, io/json.h:139-157)"""
@@ -206,7 +266,7 @@ class Sublattice:
 
 @dataclass
 class SqsConfiguration:
-    """Mirror of core/config.h:32-48 configuration<double>."""
+    """Mirror of core/config.h:32-48 configuration<T>, T = double or float (`prec`)."""
     sublattice_mode: str
     iteration_mode: str
     seed: list[Optional[int]]
@@ -269,7 +329,7 @@ def _mode(doc, key, allowed, default):
     return v
 
 
-def _parse_structure(doc) -> tuple[Structure, dict]:
+def _parse_structure(doc, single: bool = False) -> tuple[Structure, dict]:
     sc = doc.get("structure")
     if sc is None:
         raise _Fail("structure", ParseErrorCode.CODE_NOT_FOUND, "You need to specify a structure")
@@ -284,7 +344,7 @@ def _parse_structure(doc) -> tuple[Structure, dict]:
         raise _Fail("supercell", ParseErrorCode.CODE_BAD_VALUE,
                     "There must be at least one copy in each direction")
     try:
-        base = Structure(lat, coords, species)
+        base = Structure(lat, coords, species, single=single)
     except ValueError as e:
         raise _Fail("structure", ParseErrorCode.CODE_BAD_VALUE, str(e))
     st = base.supercell(*cell)
@@ -386,34 +446,35 @@ def _radii(doc, value, st: Structure) -> list[float]:
     if isinstance(value, str):
         mode = value.lower()
         dm = st.distance_matrix()
+        t = st.single
         if mode == "naive":
-            return distances_naive(dm, float(doc.get("atol", ATOL_DEFAULT)),
-                                   float(doc.get("rtol", RTOL_DEFAULT))).tolist()
+            return distances_naive(dm, _as_prec(doc.get("atol", ATOL_DEFAULT), t),
+                                   _as_prec(doc.get("rtol", RTOL_DEFAULT), t), t).tolist()
         if mode == "peak":
             try:
-                return distances_histogram(dm, float(doc.get("bin_width", BIN_WIDTH_DEFAULT)),
-                                           float(doc.get("peak_isolation",
-                                                         PEAK_ISOLATION_DEFAULT))).tolist()
+                return distances_histogram(
+                    dm, _as_prec(doc.get("bin_width", BIN_WIDTH_DEFAULT), t),
+                    _as_prec(doc.get("peak_isolation", PEAK_ISOLATION_DEFAULT), t), t).tolist()
             except ValueError as e:
                 raise _Fail("shell_radii", ParseErrorCode.CODE_BAD_VALUE, str(e))
         raise _Fail("shell_radii", ParseErrorCode.CODE_BAD_VALUE,
                     'Invalid shell radii detection mode must be either "naive" or "peak"')
-    radii = [float(r) for r in value]
+    radii = [_as_prec(r, st.single) for r in value]
     if not radii:
         raise _Fail("shell_radii", ParseErrorCode.CODE_OUT_OF_RANGE,
                     "You need to define at least one coordination shells")
     if any(r < 0 for r in radii):
         raise _Fail("shell_radii", ParseErrorCode.CODE_BAD_VALUE,
                     "You cannot specify a shell radius that is less than 0")
-    if radii[0] != 0.0 and not abs(radii[0]) <= 1e-5:
+    if radii[0] != 0.0 and not abs(radii[0]) <= _as_prec(1e-5, st.single):
         radii.insert(0, 0.0)
     return radii
 
 
-def _weights(value, num_radii: int) -> dict[int, float]:
-    """shell_weights.h:186-223"""
+def _weights(value, num_radii: int, single: bool = False) -> dict[int, float]:
+    """shell_weights.h:186-223 (defaults T(1) / T(i), shell_weights.h:75-84)"""
     if value is None:
-        return {i: 1.0 / i for i in range(1, num_radii)}
+        return {i: _as_prec(1.0 / i, single) for i in range(1, num_radii)}
     if not isinstance(value, dict):
         raise _Fail("shell_weights", ParseErrorCode.CODE_BAD_VALUE,
                     '"shell_weights" must be an object')
@@ -426,16 +487,18 @@ def _weights(value, num_radii: int) -> dict[int, float]:
         if s >= num_radii:
             raise _Fail("shell_weights", ParseErrorCode.CODE_OUT_OF_RANGE,
                         f"There are {num_radii} shells but you specified a shell index of {s}")
-        w[s] = float(v)
+        w[s] = _as_prec(v, single)
     if not w:
         raise _Fail("shell_weights", ParseErrorCode.CODE_OUT_OF_RANGE,
                     "shell weights cannot be empty")
     return dict(sorted(w.items()))
 
 
-def _array(key, value, S: int, K: int) -> np.ndarray:
+def _array(key, value, S: int, K: int, single: bool = False) -> np.ndarray:
     """arrays.h:76-101: scalar, K x K matrix stacked over shells, or S x K x K cube"""
     a = np.asarray(value, dtype=np.float64)
+    if single:  # the values of a configuration<float>
+        a = a.astype(np.float32).astype(np.float64)
     if a.ndim == 0:
         return np.full((S, K, K), float(a))
     if a.ndim == 2:
@@ -470,16 +533,14 @@ def _parse(doc: dict) -> SqsConfiguration:
         if k not in KNOWN_KEYS:
             raise _Fail(k, ParseErrorCode.CODE_UNKNOWN, f'Unknown key "{k}"')
     prec = _mode(doc, "prec", (double, single), double)
-    if prec == single:
-        raise _Fail("prec", ParseErrorCode.CODE_BAD_VALUE,
-                    "this backend evaluates objectives in FP64 only (prec \"double\")")
+    sp = prec == single  # configuration<float> (combined.h:283-295)
     it_mode = _mode(doc, "iteration_mode", (random, systematic), random)
     sl_mode = _mode(doc, "sublattice_mode", (interact, split), interact)
     if it_mode == systematic and sl_mode == split:
         raise _Fail("iteration_mode", ParseErrorCode.CODE_BAD_VALUE,
                     'It is not possible to do an exhaustive search on multiple sublattices in mode '
                     '"split"')
-    st, sc = _parse_structure(doc)
+    st, sc = _parse_structure(doc, sp)
     subs = _parse_composition(doc, st, sl_mode)
     applied = _apply_composition(st, subs)
     nprob = 1 if sl_mode == interact else len(subs)
@@ -533,9 +594,9 @@ def _parse(doc: dict) -> SqsConfiguration:
     if sl_mode == split and isinstance(wspec, (list, tuple)):
         if len(wspec) != nprob:
             raise _Fail("shell_weights", ParseErrorCode.CODE_BAD_VALUE, "Cannot interpret value")
-        weights = [_weights(w, len(r)) for w, r in zip(wspec, radii)]
+        weights = [_weights(w, len(r), sp) for w, r in zip(wspec, radii)]
     else:
-        weights = [_weights(wspec, len(r)) for r in radii]
+        weights = [_weights(wspec, len(r), sp) for r in radii]
     # arrays
     prefs, pws, tgts = [], [], []
     p_spec = _per_problem(doc, "prefactors", nprob, sl_mode)
@@ -546,12 +607,12 @@ def _parse(doc: dict) -> SqsConfiguration:
         K = len(set(structs[q].species))
         if p_spec[q] is None:  # structure.h:260-265 on the applied / decomposed structure
             sm = structs[q].shell_matrix(radii[q])
-            prefs.append(prefactors(sm, list(weights[q]), structs[q].species))
+            prefs.append(prefactors(sm, list(weights[q]), structs[q].species, sp))
         else:
-            prefs.append(_array("prefactors", p_spec[q], S, K))
-        pws.append(_array("pair_weights", w_spec[q], S, K) if w_spec[q] is not None
+            prefs.append(_array("prefactors", p_spec[q], S, K, sp))
+        pws.append(_array("pair_weights", w_spec[q], S, K, sp) if w_spec[q] is not None
                    else np.repeat((np.ones((K, K)) - np.eye(K))[None], S, axis=0))
-        tgts.append(_array("target_objective", t_spec[q], S, K) if t_spec[q] is not None
+        tgts.append(_array("target_objective", t_spec[q], S, K, sp) if t_spec[q] is not None
                     else np.zeros((S, K, K)))
     # chunk size, threads, keep, max results
     chunk = int(doc.get("chunk_size", min(iterations, CHUNK_SIZE_DEFAULT)))
@@ -633,7 +694,7 @@ class _Arrays:
             L.ptr(self.weights, L.f64p), L.ptr(self.K, L.u32p), L.ptr(self.pref, L.f64p),
             L.ptr(self.pw, L.f64p), L.ptr(self.tgt, L.f64p), cfg.iterations, cfg.chunk_size,
             L.ptr(self.seeds, L.u64p), L.ptr(self.seed_present, L.u8p), cfg.keep,
-            -1 if mrpo is None else mrpo)
+            -1 if mrpo is None else mrpo, 1 if cfg.prec == single else 0)
 
 
 class Job:
@@ -1163,8 +1224,8 @@ def load_result_pack(data, prec: str = double) -> StoredResultPack:
     """python/extension/sqsgen.cpp:38-56,528-531: a msgpack result pack from a file path or
     from the bytes themselves"""
     from . import pack_io as W
-    if prec != double:
-        raise ValueError("this backend keeps results in FP64 only (prec \"double\")")
+    if prec not in (double, single):
+        raise ValueError(f"unknown precision {prec!r}")
     return StoredResultPack.from_bytes(W.read_pack_input(data))
 
 

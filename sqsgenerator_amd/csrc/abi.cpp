@@ -56,12 +56,6 @@ void copy_str(const std::string &s, char *out, size_t cap) {
   std::memcpy(out, s.c_str(), s.size() + 1);
 }
 
-sqsg::Lattice lattice(const double *m) {
-  sqsg::Lattice l;
-  std::memcpy(l.m, m, sizeof(l.m));
-  return l;
-}
-
 }  // namespace
 
 namespace sqsg {
@@ -88,67 +82,132 @@ int sqsg_device_info(char *name, size_t cap, int *compute_units) {
 
 int sqsg_device_synchronize(void) { return guard([] { SQSG_HIP(hipDeviceSynchronize()); }); }
 
-int sqsg_distance_matrix(const double *lat, const double *frac, uint32_t n, double *out) {
+}  // extern "C"
+
+namespace {
+// the geometry entry points for T = double and T = float (structure<T>)
+template <class T> int distance_matrix_t(const T *lat, const T *frac, uint32_t n, T *out) {
   return guard([&] {
     need(lat, "lattice"), need(frac, "frac"), need(out, "out");
-    const auto d = sqsg::distance_matrix(lattice(lat), frac, n);
-    std::memcpy(out, d.data(), d.size() * sizeof(double));
+    sqsg::LatticeT<T> L;
+    std::memcpy(L.m, lat, sizeof(L.m));
+    const auto d = sqsg::distance_matrix<T>(L, frac, n);
+    std::memcpy(out, d.data(), d.size() * sizeof(T));
   });
 }
 
-int sqsg_distances_naive(const double *dm, uint32_t n, double atol, double rtol, double *out,
-                         uint32_t cap, uint32_t *count) {
+template <class T>
+int distances_naive_t(const T *dm, uint32_t n, T atol, T rtol, T *out, uint32_t cap,
+                      uint32_t *count) {
   return guard([&] {
     need(dm, "dm"), need(count, "count");
-    const auto d = sqsg::distances_naive(std::vector<double>(dm, dm + size_t(n) * n), n, atol, rtol);
+    const auto d = sqsg::distances_naive<T>(std::vector<T>(dm, dm + size_t(n) * n), n, atol, rtol);
     *count = static_cast<uint32_t>(d.size());
-    if (out) std::memcpy(out, d.data(), std::min<size_t>(cap, d.size()) * sizeof(double));
+    if (out) std::memcpy(out, d.data(), std::min<size_t>(cap, d.size()) * sizeof(T));
   });
 }
 
-int sqsg_distances_histogram(const double *dm, uint32_t n, double bin_width,
-                             double peak_isolation, double *out, uint32_t cap, uint32_t *count) {
+template <class T>
+int distances_histogram_t(const T *dm, uint32_t n, T bin_width, T peak_isolation, T *out,
+                          uint32_t cap, uint32_t *count) {
   return guard([&] {
     need(dm, "dm"), need(count, "count");
-    const auto d = sqsg::distances_histogram(std::vector<double>(dm, dm + size_t(n) * n), n,
-                                             bin_width, peak_isolation);
+    const auto d = sqsg::distances_histogram<T>(std::vector<T>(dm, dm + size_t(n) * n), n,
+                                                bin_width, peak_isolation);
     *count = static_cast<uint32_t>(d.size());
-    if (out) std::memcpy(out, d.data(), std::min<size_t>(cap, d.size()) * sizeof(double));
+    if (out) std::memcpy(out, d.data(), std::min<size_t>(cap, d.size()) * sizeof(T));
   });
 }
 
-int sqsg_shell_matrix(const double *dm, uint32_t n, const double *radii, uint32_t num_radii,
-                      double atol, double rtol, uint32_t *out) {
+template <class T>
+int shell_matrix_t(const T *dm, uint32_t n, const T *radii, uint32_t num_radii, T atol, T rtol,
+                   uint32_t *out) {
   return guard([&] {
     need(dm, "dm"), need(radii, "radii"), need(out, "out");
-    const auto sm = sqsg::shell_matrix(std::vector<double>(dm, dm + size_t(n) * n), n,
-                                       std::vector<double>(radii, radii + num_radii), atol, rtol);
+    const auto sm = sqsg::shell_matrix<T>(std::vector<T>(dm, dm + size_t(n) * n), n,
+                                          std::vector<T>(radii, radii + num_radii), atol, rtol);
     std::memcpy(out, sm.data(), sm.size() * sizeof(uint32_t));
   });
 }
 
-int sqsg_prefactors(const uint32_t *sm, uint32_t n, const uint32_t *shells, uint32_t num_shells,
-                    const uint8_t *species_z, double *out) {
+template <class T>
+int prefactors_t(const uint32_t *sm, uint32_t n, const uint32_t *shells, uint32_t num_shells,
+                 const uint8_t *species_z, T *out) {
   return guard([&] {
     need(sm, "shell_matrix"), need(shells, "shells"), need(species_z, "species"), need(out, "out");
     uint32_t K = 0;
-    const auto p = sqsg::prefactors(std::vector<uint32_t>(sm, sm + size_t(n) * n), n,
-                                    std::vector<uint32_t>(shells, shells + num_shells),
-                                    std::vector<uint8_t>(species_z, species_z + n), &K);
-    std::memcpy(out, p.data(), p.size() * sizeof(double));
+    const auto p = sqsg::prefactors<T>(std::vector<uint32_t>(sm, sm + size_t(n) * n), n,
+                                       std::vector<uint32_t>(shells, shells + num_shells),
+                                       std::vector<uint8_t>(species_z, species_z + n), &K);
+    std::memcpy(out, p.data(), p.size() * sizeof(T));
   });
 }
 
-int sqsg_supercell(const double *lat, const double *frac, const uint8_t *species, uint32_t n,
-                   uint32_t a, uint32_t b, uint32_t c, double *lattice_out, double *frac_out,
-                   uint8_t *species_out) {
+template <class T>
+int supercell_t(const T *lat, const T *frac, const uint8_t *species, uint32_t n, uint32_t a,
+                uint32_t b, uint32_t c, T *lattice_out, T *frac_out, uint8_t *species_out) {
   return guard([&] {
     need(lat, "lattice"), need(frac, "frac"), need(species, "species");
     need(lattice_out, "lattice_out"), need(frac_out, "frac_out"), need(species_out, "species_out");
-    sqsg::Lattice lo;
-    sqsg::supercell(lattice(lat), frac, species, n, a, b, c, &lo, frac_out, species_out);
+    sqsg::LatticeT<T> L, lo;
+    std::memcpy(L.m, lat, sizeof(L.m));
+    sqsg::supercell<T>(L, frac, species, n, a, b, c, &lo, frac_out, species_out);
     std::memcpy(lattice_out, lo.m, sizeof(lo.m));
   });
+}
+}  // namespace
+
+extern "C" {
+
+int sqsg_distance_matrix(const double *lat, const double *frac, uint32_t n, double *out) {
+  return distance_matrix_t<double>(lat, frac, n, out);
+}
+int sqsg_distances_naive(const double *dm, uint32_t n, double atol, double rtol, double *out,
+                         uint32_t cap, uint32_t *count) {
+  return distances_naive_t<double>(dm, n, atol, rtol, out, cap, count);
+}
+int sqsg_distances_histogram(const double *dm, uint32_t n, double bin_width,
+                             double peak_isolation, double *out, uint32_t cap, uint32_t *count) {
+  return distances_histogram_t<double>(dm, n, bin_width, peak_isolation, out, cap, count);
+}
+int sqsg_shell_matrix(const double *dm, uint32_t n, const double *radii, uint32_t num_radii,
+                      double atol, double rtol, uint32_t *out) {
+  return shell_matrix_t<double>(dm, n, radii, num_radii, atol, rtol, out);
+}
+int sqsg_prefactors(const uint32_t *sm, uint32_t n, const uint32_t *shells, uint32_t num_shells,
+                    const uint8_t *species_z, double *out) {
+  return prefactors_t<double>(sm, n, shells, num_shells, species_z, out);
+}
+int sqsg_supercell(const double *lat, const double *frac, const uint8_t *species, uint32_t n,
+                   uint32_t a, uint32_t b, uint32_t c, double *lattice_out, double *frac_out,
+                   uint8_t *species_out) {
+  return supercell_t<double>(lat, frac, species, n, a, b, c, lattice_out, frac_out, species_out);
+}
+
+int sqsg_distance_matrix_f32(const float *lat, const float *frac, uint32_t n, float *out) {
+  return distance_matrix_t<float>(lat, frac, n, out);
+}
+int sqsg_distances_naive_f32(const float *dm, uint32_t n, float atol, float rtol, float *out,
+                             uint32_t cap, uint32_t *count) {
+  return distances_naive_t<float>(dm, n, atol, rtol, out, cap, count);
+}
+int sqsg_distances_histogram_f32(const float *dm, uint32_t n, float bin_width,
+                                 float peak_isolation, float *out, uint32_t cap,
+                                 uint32_t *count) {
+  return distances_histogram_t<float>(dm, n, bin_width, peak_isolation, out, cap, count);
+}
+int sqsg_shell_matrix_f32(const float *dm, uint32_t n, const float *radii, uint32_t num_radii,
+                          float atol, float rtol, uint32_t *out) {
+  return shell_matrix_t<float>(dm, n, radii, num_radii, atol, rtol, out);
+}
+int sqsg_prefactors_f32(const uint32_t *sm, uint32_t n, const uint32_t *shells,
+                        uint32_t num_shells, const uint8_t *species_z, float *out) {
+  return prefactors_t<float>(sm, n, shells, num_shells, species_z, out);
+}
+int sqsg_supercell_f32(const float *lat, const float *frac, const uint8_t *species, uint32_t n,
+                       uint32_t a, uint32_t b, uint32_t c, float *lattice_out, float *frac_out,
+                       uint8_t *species_out) {
+  return supercell_t<float>(lat, frac, species, n, a, b, c, lattice_out, frac_out, species_out);
 }
 
 int sqsg_num_permutations(const uint8_t *conf, uint32_t n, char *out, size_t cap) {

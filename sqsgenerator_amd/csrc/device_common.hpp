@@ -34,6 +34,26 @@ __device__ __forceinline__ double absolute(double a) { return a > 0 ? a : -a; } 
 // the following multiply as a source modifier instead of a compare and select.
 __device__ __forceinline__ double term_abs(double a) { return __builtin_fabs(a); }
 
+// One objective term (optimization.h:85-96): v = 1 - count * pref, pw * |v - tgt|, in double,
+// or in float for a configuration<float> (`single`; the parameters then hold float values).
+// Returns the term and sets v (the SRO entry), both as double.
+__device__ __forceinline__ double term_value(bool single, uint32_t count, double pref, double pw,
+                                             double tgt, double &v) {
+  if (single) {
+    const float vf = 1.0f - static_cast<float>(count) * static_cast<float>(pref);
+    const float d = vf - static_cast<float>(tgt);
+    v = static_cast<double>(vf);
+    return static_cast<double>(static_cast<float>(pw) * (d > 0 ? d : -d));
+  }
+  v = 1.0 - static_cast<double>(count) * pref;
+  return pw * absolute(v - tgt);
+}
+
+// objective += term in the configuration's type
+__device__ __forceinline__ double add_term(bool single, double obj, double t) {
+  return single ? static_cast<double>(static_cast<float>(obj) + static_cast<float>(t)) : obj + t;
+}
+
 __device__ __forceinline__ unsigned long long okey(double d) {
   const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(d));
   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);

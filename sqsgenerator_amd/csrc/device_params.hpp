@@ -53,6 +53,7 @@ struct DevProblem {
   uint32_t nbounds;
   uint32_t bounds[2 * kMaxBounds];
   uint32_t draws;             // RNG draws per candidate: sum over windows of (w - 1)
+  uint32_t single;            // configuration<float>: objective terms and sums in float
   uint32_t row_stride;        // bytes of one LDS candidate row (N rounded up, bank-padded)
   uint64_t seed;
   // systematic enumeration of window bounds[0..1]

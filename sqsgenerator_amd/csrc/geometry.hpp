@@ -10,35 +10,46 @@
 
 namespace sqsg {
 
-struct Lattice {
-  double m[9];  // row-major, rows are the cell vectors a, b, c
+template <class T> struct LatticeT {
+  T m[9];  // row-major, rows are the cell vectors a, b, c
 };
+using Lattice = LatticeT<double>;
+
+// Every function exists for T = double and T = float (configuration<double> and
+// configuration<float>): the reference evaluates the whole geometry in the configuration's
+// precision.
 
 // structure.h:34-69: minimum-image distances over the 27 neighbouring cells
-std::vector<double> distance_matrix(const Lattice &lat, const double *frac, uint32_t n);
+template <class T>
+std::vector<T> distance_matrix(const LatticeT<T> &lat, const T *frac, uint32_t n);
 
 // structure.h:195-207
-std::vector<double> distances_naive(const std::vector<double> &dm, uint32_t n, double atol,
-                                    double rtol);
+template <class T>
+std::vector<T> distances_naive(const std::vector<T> &dm, uint32_t n, T atol, T rtol);
 
 // structure.h:209-258 (throws std::invalid_argument below 10 histogram edges)
-std::vector<double> distances_histogram(const std::vector<double> &dm, uint32_t n,
-                                        double bin_width, double peak_isolation);
+template <class T>
+std::vector<T> distances_histogram(const std::vector<T> &dm, uint32_t n, T bin_width,
+                                   T peak_isolation);
 
 // structure.h:71-100
-std::vector<uint32_t> shell_matrix(const std::vector<double> &dm, uint32_t n,
-                                   const std::vector<double> &radii, double atol, double rtol);
+template <class T>
+std::vector<uint32_t> shell_matrix(const std::vector<T> &dm, uint32_t n,
+                                   const std::vector<T> &radii, T atol, T rtol);
 
 // structure.h:136-192; shells ascending; species are atomic numbers
-std::vector<double> prefactors(const std::vector<uint32_t> &sm, uint32_t n,
-                               const std::vector<uint32_t> &shells,
-                               const std::vector<uint8_t> &species, uint32_t *num_species);
+template <class T>
+std::vector<T> prefactors(const std::vector<uint32_t> &sm, uint32_t n,
+                          const std::vector<uint32_t> &shells,
+                          const std::vector<uint8_t> &species, uint32_t *num_species);
 
 // structure.h:335-366
-void supercell(const Lattice &lat, const double *frac, const uint8_t *species, uint32_t n,
-               uint32_t a, uint32_t b, uint32_t c, Lattice *lat_out, double *frac_out,
+template <class T>
+void supercell(const LatticeT<T> &lat, const T *frac, const uint8_t *species, uint32_t n,
+               uint32_t a, uint32_t b, uint32_t c, LatticeT<T> *lat_out, T *frac_out,
                uint8_t *species_out);
 
 constexpr double kEps = 2.220446049250313e-16;  // std::numeric_limits<double>::epsilon()
+constexpr float kEpsF = 1.1920929e-7f;            // std::numeric_limits<float>::epsilon()
 
 }  // namespace sqsg

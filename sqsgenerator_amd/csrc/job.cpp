@@ -21,7 +21,7 @@ namespace sqsg {
 const char *const kDenseDomain =
     "the dense kernel needs problems of at most 1024 sites, 16 species and 127 upper "
     "neighbours per site and shell (split jobs: Monte Carlo, at most 256 sites per "
-    "sublattice)";
+    "sublattice) and a configuration in double precision";
 
 void hip_check(hipError_t e, const char *what) {
   if (e == hipErrorOutOfMemory) throw std::bad_alloc();
@@ -112,8 +112,18 @@ void build_problem(Problem &P, const ProblemInput &in, const Lattice &lat, const
   std::vector<double> weights(c.shell_weights + s0, c.shell_weights + s1);
   std::vector<double> radii(c.radii + c.radii_offsets[p], c.radii + c.radii_offsets[p + 1]);
   // pair list of the sorted structure (structure.h:434-451; optimization_config.h:134)
-  const std::vector<double> dm = distance_matrix(lat, in.frac.data(), n);
-  const std::vector<uint32_t> sm = shell_matrix(dm, n, radii, kEps, 1.0e-9);
+  std::vector<uint32_t> sm;
+  if (c.precision == SQSG_PREC_SINGLE) {  // structure<float>: the geometry in float
+    LatticeT<float> latf;
+    for (int k = 0; k < 9; ++k) latf.m[k] = static_cast<float>(lat.m[k]);
+    const std::vector<float> fracf(in.frac.begin(), in.frac.end());
+    const std::vector<float> radf(radii.begin(), radii.end());
+    const std::vector<float> dm = distance_matrix<float>(latf, fracf.data(), n);
+    sm = shell_matrix<float>(dm, n, radf, kEpsF, 1.0e-9f);
+  } else {
+    const std::vector<double> dm = distance_matrix<double>(lat, in.frac.data(), n);
+    sm = shell_matrix<double>(dm, n, radii, kEps, 1.0e-9);
+  }
   std::map<uint32_t, uint32_t> shell_index;
   for (uint32_t s = 0; s < P.S; ++s) shell_index[P.shell_ids[s]] = s;
   for (uint32_t i = 0; i < n; ++i)
@@ -135,12 +145,17 @@ void build_problem(Problem &P, const ProblemInput &in, const Lattice &lat, const
   P.pw.assign(c.pair_weights + cube_off, c.pair_weights + cube_off + cube);
   P.tgt.assign(c.target_objective + cube_off, c.target_objective + cube_off + cube);
   const uint32_t K = P.K;
+  P.single = c.precision == SQSG_PREC_SINGLE;
+  // w *= w_s in the configuration's type (a float product is the double product rounded)
+  auto scale = [&](double x, double w) {
+    return P.single ? static_cast<double>(static_cast<float>(x) * static_cast<float>(w)) : x * w;
+  };
   for (uint32_t s = 0; s < P.S; ++s)
     for (uint32_t xi = 0; xi < K; ++xi)
       for (uint32_t eta = xi; eta < K; ++eta) {
-        P.pw[(s * K + xi) * K + eta] *= weights[s];
+        P.pw[(s * K + xi) * K + eta] = scale(P.pw[(s * K + xi) * K + eta], weights[s]);
         if (xi == eta) continue;
-        P.pw[(s * K + eta) * K + xi] *= weights[s];
+        P.pw[(s * K + eta) * K + xi] = scale(P.pw[(s * K + eta) * K + xi], weights[s]);
       }
   // objective terms in the order of optimization.h:85-100
   for (uint32_t s = 0; s < P.S; ++s)
@@ -339,7 +354,7 @@ void analyze_job(sqsg_job *job) {
   }
   // one problem of up to 1024 sites, or the sublattice problems of a split job (a form for
   // Monte Carlo runs up to the tile bound dense_split_available reports: 256 sites each)
-  job->dense_ok = true;
+  job->dense_ok = !job->single;  // the dense kernels evaluate FP64 objectives only
   for (const auto &P : job->problems)
     job->dense_ok = job->dense_ok && P.N <= 1024 && P.K <= 16 && P.max_upper <= 127;
   if (job->problems.size() > 1)
@@ -377,6 +392,7 @@ DevProblem Problem::dev() const {
   d.tgt = d_tgt.as<const double>();
   d.terms = d_terms.as<const uint32_t>();
   d.tparams = d_tparams.as<const TermParam>();
+  d.single = single ? 1u : 0u;
   d.lane_stream = d_lane_stream.as<const uint32_t>();
   d.lane_off = d_lane_off.as<const uint32_t>();
   d.species0F = d_species0F.as<const uint8_t>();
@@ -431,6 +447,9 @@ sqsg_job *create_job(const sqsg_config &c) {
   }
   job->keep = c.keep;
   job->max_results = c.max_results_per_objective;
+  if (c.precision != SQSG_PREC_DOUBLE && c.precision != SQSG_PREC_SINGLE)
+    throw std::runtime_error("Unsupported precision");
+  job->single = c.precision == SQSG_PREC_SINGLE;
   Lattice lat;
   std::memcpy(lat.m, c.lattice, sizeof(lat.m));
   // apply_composition (structure.h:390-399): sites in ascending order, species by Z

@@ -61,6 +61,7 @@ class DevBuf {
 // One sublattice problem (interact: the whole structure; split: one sublattice).
 struct Problem {
   uint32_t N = 0, K = 0, S = 0;
+  bool single = false;                  // configuration<float>
   std::vector<uint32_t> pi, pj, ps;     // pairs in the reference's build order (i, then j)
   std::vector<uint8_t> species_packed;  // sorted-site order, packed indices
   std::vector<uint32_t> sort_order;     // sorted position -> site of the problem's structure
@@ -162,6 +163,7 @@ struct ResultSet {
 
 struct sqsg_job {
   int iteration_mode = 0, sublattice_mode = 0;
+  bool single = false;  // configuration<float>: float geometry, weights and objectives
   std::vector<sqsg::Problem> problems;
   uint32_t num_sites = 0;
   std::vector<uint8_t> structure_z;  // applied composition, original order

@@ -66,8 +66,8 @@ __device__ __forceinline__ double objective_wave(const DevProblem &pr, const uin
     const uint32_t ix = (s * K + xi) * K + eta;
     uint32_t cnt = hist[ix];
     if (xi != eta) cnt += hist[(s * K + eta) * K + xi];
-    const double v = 1.0 - static_cast<double>(cnt) * pr.pref[ix];
-    terms[t] = pr.pw[ix] * absolute(v - pr.tgt[ix]);
+    double v;
+    terms[t] = term_value(pr.single != 0, cnt, pr.pref[ix], pr.pw[ix], pr.tgt[ix], v);
     if (sro) {
       sro[ix] = v;
       if (xi != eta) sro[(s * K + eta) * K + xi] = v;
@@ -76,7 +76,7 @@ __device__ __forceinline__ double objective_wave(const DevProblem &pr, const uin
   wave_sync();
   double obj = 0.0;
   if (lane == 0)
-    for (uint32_t t = 0; t < pr.T; ++t) obj += terms[t];
+    for (uint32_t t = 0; t < pr.T; ++t) obj = add_term(pr.single != 0, obj, terms[t]);
   obj = __shfl(obj, 0);
   return obj;
 }
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void run_wave_kernel(RunParams p) {
           tot = sub[0];
         } else {  // helpers::sum folds from T(0) (core/helpers/fold.h:51-53)
           tot = 0.0;
-          for (uint32_t q = 0; q < p.nprob; ++q) tot += sub[q];
+          for (uint32_t q = 0; q < p.nprob; ++q) tot = add_term(p.prob[0].single != 0, tot, sub[q]);
         }
         ++scored;
         const unsigned long long key = okey(tot);

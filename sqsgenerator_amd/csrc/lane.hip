@@ -97,19 +97,22 @@ __device__ double score_lane(const uint8_t *col, const DevProblem &pr, uint32_t 
       }
     }
     // this shell's objective terms, optimization.h:85-100
+    const bool single = pr.single != 0;
     const double *pref = pr.pref + s * K * K;
     const double *pw = pr.pw + s * K * K;
     const double *tgt = pr.tgt + s * K * K;
     for (uint32_t xi = 0; xi < K; ++xi) {
       const uint32_t cxx = field_count<KM>(hl, xi, xi);
-      const double sxx = 1.0 - static_cast<double>(cxx) * pref[xi * K + xi];
-      obj += pw[xi * K + xi] * absolute(sxx - tgt[xi * K + xi]);
+      double sxx;
+      obj = add_term(single, obj,
+                     term_value(single, cxx, pref[xi * K + xi], pw[xi * K + xi], tgt[xi * K + xi], sxx));
       if (sro) sro[(s * K + xi) * K + xi] = sxx;
       if (bonds) bonds[(s * K + xi) * K + xi] = static_cast<int32_t>(cxx);
       for (uint32_t eta = xi + 1; eta < K; ++eta) {
         const uint32_t cab = field_count<KM>(hl, xi, eta), cba = field_count<KM>(hl, eta, xi);
-        const double v = 1.0 - static_cast<double>(cab + cba) * pref[xi * K + eta];
-        obj += pw[xi * K + eta] * absolute(v - tgt[xi * K + eta]);
+        double v;
+        obj = add_term(single, obj, term_value(single, cab + cba, pref[xi * K + eta],
+                                               pw[xi * K + eta], tgt[xi * K + eta], v));
         if (sro) sro[(s * K + xi) * K + eta] = v, sro[(s * K + eta) * K + xi] = v;
         if (bonds) {
           bonds[(s * K + xi) * K + eta] = static_cast<int32_t>(cab);
@@ -166,7 +169,7 @@ __device__ void lane_segment(const RunParams &p, uint64_t sbeg, uint64_t end, ui
       tot = sub[0];
     } else {  // helpers::sum folds from T(0) (core/helpers/fold.h:51-53)
       tot = 0.0;
-      for (uint32_t q = 0; q < p.nprob; ++q) tot += sub[q];
+      for (uint32_t q = 0; q < p.nprob; ++q) tot = add_term(p.prob[0].single != 0, tot, sub[q]);
     }
     scored += static_cast<unsigned long long>(__popcll(__ballot(active)));
     const unsigned long long g =
