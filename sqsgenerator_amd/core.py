@@ -1226,7 +1226,10 @@ def load_result_pack(data, prec: str = double) -> StoredResultPack:
     from . import pack_io as W
     if prec not in (double, single):
         raise ValueError(f"unknown precision {prec!r}")
-    return StoredResultPack.from_bytes(W.read_pack_input(data))
+    pack = StoredResultPack.from_bytes(W.read_pack_input(data))
+    # the wire form carries no precision (io/json.h:177-195): the caller names it
+    pack.config.prec = prec
+    return pack
 
 
 def optimize(config: Union[dict, SqsConfiguration, str], level: Any = None,

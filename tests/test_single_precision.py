@@ -232,3 +232,22 @@ def test_float_job_batched_with_double_job(gpu):
     FloatRescorer(cf).check_pack(got)
     for j in jobs:
         j.close()
+
+
+@pytest.mark.gpu
+def test_float_pack_round_trip(gpu):
+    """a float run's pack through the wire form and load_result_pack(..., "single")"""
+    cfg = core.parse_config(single(W.ternary_108(iterations=100_000)))
+    job = core.Job(cfg)
+    job.run()
+    pack = job.results()
+    loaded = core.load_result_pack(pack.bytes(), "single")
+    assert loaded.config.prec == "single"
+    la, lb = list(pack), list(loaded)
+    assert [o for o, _ in la] == [o for o, _ in lb]
+    assert all(np.float64(np.float32(o)) == o for o, _ in lb)
+    for (_, ra), (_, rb) in zip(la, lb):
+        sa = sorted((tuple(r.species), tuple(r.sro().ravel())) for r in ra)
+        sb = sorted((tuple(r.species), tuple(r.sro().ravel())) for r in rb)
+        assert sa == sb
+    job.close()
