@@ -75,8 +75,15 @@ __device__ __forceinline__ double objective_wave(const DevProblem &pr, const uin
   }
   wave_sync();
   double obj = 0.0;
-  if (lane == 0)
-    for (uint32_t t = 0; t < pr.T; ++t) obj = add_term(pr.single != 0, obj, terms[t]);
+  if (lane == 0) {
+    if (pr.single) {  // configuration<float>: the sum in float
+      float o = 0.0f;
+      for (uint32_t t = 0; t < pr.T; ++t) o += static_cast<float>(terms[t]);
+      obj = o;
+    } else {
+      for (uint32_t t = 0; t < pr.T; ++t) obj += terms[t];
+    }
+  }
   obj = __shfl(obj, 0);
   return obj;
 }

@@ -50,9 +50,11 @@ __device__ __forceinline__ uint32_t field_count(const uint32_t *hl, uint32_t a, 
 
 // Bond histogram + objective of the lane's candidate (`col`), shell by shell.  `hl` is
 // the lane's histogram column (hist + lane).  Optionally writes ordered bonds and SRO.
-template <int KM>
-__device__ double score_lane(const uint8_t *col, const DevProblem &pr, uint32_t *hl,
-                             int32_t *bonds, double *sro) {
+// SINGLE: the terms and their sum in float (configuration<float>), fixed at compile time so
+// that the double path carries no per-term selection.
+template <int KM, bool SINGLE>
+__device__ double score_lane_t(const uint8_t *col, const DevProblem &pr, uint32_t *hl,
+                               int32_t *bonds, double *sro) {
   using Sh = LaneShape<KM>;
   constexpr uint32_t F = Sh::F, W = Sh::W, MASK = Sh::MASK;
   const uint32_t K = pr.K;
@@ -97,7 +99,7 @@ __device__ double score_lane(const uint8_t *col, const DevProblem &pr, uint32_t 
       }
     }
     // this shell's objective terms, optimization.h:85-100
-    const bool single = pr.single != 0;
+    constexpr bool single = SINGLE;
     const double *pref = pr.pref + s * K * K;
     const double *pw = pr.pw + s * K * K;
     const double *tgt = pr.tgt + s * K * K;
@@ -122,6 +124,13 @@ __device__ double score_lane(const uint8_t *col, const DevProblem &pr, uint32_t 
     }
   }
   return obj;
+}
+
+template <int KM>
+__device__ __forceinline__ double score_lane(const uint8_t *col, const DevProblem &pr,
+                                             uint32_t *hl, int32_t *bonds, double *sro) {
+  return pr.single ? score_lane_t<KM, true>(col, pr, hl, bonds, sro)
+                   : score_lane_t<KM, false>(col, pr, hl, bonds, sro);
 }
 
 __device__ __forceinline__ void init_rows(const RunParams &p, uint8_t *wl, int lane) {
